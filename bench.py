@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Benchmark: plain APSM train + detect on one MI355X.
+
+Default workload (16 antennas, 10 superimposed 16QAM users with large-scale gains
+uniform in [-6, 0] dB, 15 dB SNR; all 10 users detected; windowSize 160; dictionary
+capped at 4096 entries; 200k pilots and 2M detect samples): one step = reset, train on
+the pilots and detect all data samples for all users, inputs resident in HBM.
+Metric: detected symbols/s = data samples x users / step time.
+
+Prints ONE JSON line. `roofline` covers the dominant kernel (by device time) with its
+algorithmic FLOPs; `cpu_baseline` times the FP32 restatement of the reference algorithm
+(the reference's own schedule) on host cores over a bounded sample and scales it to the
+whole workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="plain-16ant-16qam")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-lls", action="store_true")
+    return ap.parse_args()
+
+
+def train_flops_per_user(W2, H, N, R2, q=2):
+    """Algorithmic FLOPs of the incremental train for one user: each entering complex row
+    against the dictionary (8N cross + 16 per complex pair) and the window Gram product
+    (2 W^2 per step)."""
+    total = 0.0
+    steps = R2 // q
+    s = np.arange(1, steps + 1, dtype=np.float64) * q
+    G = np.minimum(s - q, H) / 2.0  # complex dictionary entries seen by the entering row
+    W = np.minimum(W2, s)
+    total = np.sum((q / 2.0) * G * (8.0 * N + 16.0) + 2.0 * W * W)
+    return float(total)
+
+
+def naive_evals(W2, H, R2, q=2):
+    """Real kernel evaluations of the reference schedule for one user: every step evaluates
+    every window row against every dictionary row (apsm_train.cu:333-356)."""
+    s = np.arange(1, R2 // q + 1, dtype=np.float64) * q
+    return float(np.sum(np.minimum(W2, s) * np.minimum(s, H)))
+
+
+def cpu_baseline(w, data, budget_s):
+    from libapsm_amd import datagen
+    from oracle import pyoracle
+
+    cfg = {k: v for k, v in datagen.workload_config(w).items() if k != "otype"}
+    N = w["N"]
+    W2, H = 2 * w["window"], 2 * w["dict_size"]
+    threads = pyoracle.threads()
+    # calibrate: a short naive train on the first pilots, then detect a few samples
+    Ts = 256
+    while True:
+        o = pyoracle.Apsm(cfg, N, fp32=True)
+        t0 = time.perf_counter()
+        o.train(data["rx_train"][:, :Ts], data["tx_train"][0, :Ts])
+        t_train = time.perf_counter() - t0
+        if t_train > budget_s / 4 or Ts >= 8192:
+            break
+        Ts *= 2
+    ev_train = naive_evals(W2, H, 2 * Ts)
+    Ms = 512
+    while True:
+        t0 = time.perf_counter()
+        o.detect(data["rx_data"][:, :Ms])
+        t_det = time.perf_counter() - t0
+        if t_det > budget_s / 4 or Ms >= 65536:
+            break
+        Ms *= 2
+    Gs = min(2 * Ts, H)
+    ev_det = 2.0 * Ms * Gs
+    per_train_eval = t_train / ev_train
+    per_det_eval = t_det / ev_det
+    U = len(w["users"])
+    full_train = naive_evals(W2, H, 2 * w["T"]) * per_train_eval
+    full_det = 2.0 * w["M"] * min(2 * w["T"], H) * per_det_eval
+    total = U * (full_train + full_det)
+    return {
+        "value": w["M"] * U / total,
+        "unit": "detected symbols/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"FP32 restatement, reference schedule; user 0 trained on the first {Ts} pilots "
+                   f"({t_train:.2f} s) and {Ms} samples detected ({t_det:.2f} s); per-kernel-evaluation "
+                   f"times scaled to {U} users x ({w['T']} pilots, {w['M']} samples): "
+                   f"train {U * full_train:.0f} s + detect {U * full_det:.0f} s"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    from libapsm_amd import ApsmNomaDetectorFactory, LlsNomaDetector, datagen
+
+    w = datagen.WORKLOADS[args.workload]
+    N, U, T, M = w["N"], len(w["users"]), w["T"], w["M"]
+    data = datagen.generate(N, w["U_tx"], T, M, modulation=w["modulation"], snr_db=w["snr_db"], seed=w["seed"],
+                            large_scale_db=w["large_scale_db"])
+    tx_train = np.ascontiguousarray(data["tx_train"][w["users"]])
+    dev = torch.device("cuda:0")
+
+    def to_dev(z):
+        return torch.from_numpy(np.ascontiguousarray(z).view(np.float32)).to(dev)
+
+    d_rxt, d_txt, d_rxd = to_dev(data["rx_train"]), to_dev(tx_train), to_dev(data["rx_data"])
+    d_est = torch.empty((U, M, 2), dtype=torch.float32, device=dev)
+    cfg = datagen.workload_config(w)
+    det = ApsmNomaDetectorFactory.build(cfg, N, U)
+    torch.cuda.synchronize()
+
+    def step():
+        det.resetState()
+        det.train_device(d_rxt.data_ptr(), d_txt.data_ptr(), T)
+        det.detect_device(d_rxd.data_ptr(), M, d_est.data_ptr())
+
+    for _ in range(args.warmup):
+        step()
+    det.synchronize()
+    torch.cuda.synchronize()
+    t_train_k, t_det_k = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        a, b = det.kernel_times()
+        t_train_k.append(a)
+        t_det_k.append(b)
+    det.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_step = 1000.0 * elapsed / args.steps
+    value = M * U / (ms_step / 1000.0)
+
+    est = d_est.cpu().numpy()
+    est_c = est[..., 0] + 1j * est[..., 1]
+    tx_data = data["tx_data"][w["users"]]
+    hd = datagen.hard_decision(est_c, data["bps"])
+    ser = float(np.mean(hd != tx_data))
+
+    tr_ms, dt_ms = float(np.mean(t_train_k)), float(np.mean(t_det_k))
+    W2, H = 2 * w["window"], 2 * w["dict_size"]
+    train_flops = U * train_flops_per_user(W2, H, N, 2 * T)
+    G = min(T, w["dict_size"])
+    det_flops = M * G * (8.0 * N + 8.0 * U) + M * U * 8.0 * N
+    kern = {
+        "apsm_train_seq": dict(ms=tr_ms, flops=train_flops),
+        "apsm_detect_kernel": dict(ms=dt_ms, flops=det_flops),
+    }
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    ach = kern[dom]["flops"] / (kern[dom]["ms"] / 1000.0) / 1e12
+    roofline = {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / PEAK_FP32_TFLOPS, "traffic": None}
+    other = "apsm_detect_kernel" if dom == "apsm_train_seq" else "apsm_train_seq"
+    ach2 = kern[other]["flops"] / (kern[other]["ms"] / 1000.0) / 1e12
+
+    out = {
+        "metric": "detected symbols/s (plain APSM train+detect, all users)",
+        "value": value,
+        "unit": "detected symbols/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded Rayleigh NOMA uplink, see libapsm_amd/datagen.py)",
+        "config": {"workload": args.workload, "antennas": N, "users_detected": U, "pilots": T, "detect_samples": M,
+                   "modulation": w["modulation"], "snr_db": w["snr_db"], "windowSize": w["window"],
+                   "dictionarySize": w["dict_size"], "parallelism": "single-gpu"},
+        "roofline": roofline,
+        "kernels_ms": {"train_seq": tr_ms, "detect": dt_ms},
+        "secondary_roofline": {"kernel": other, "achieved": ach2, "unit": "TFLOP/s", "frac": ach2 / PEAK_FP32_TFLOPS},
+        "ser": ser,
+    }
+    if not args.no_lls:
+        lls = LlsNomaDetector(N)
+        d_txu = to_dev(data["tx_train"][w["users"][0]])
+        d_el = torch.empty((M, 2), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lls.train_device(d_rxt.data_ptr(), d_txu.data_ptr(), T)
+        lls.detect_device(d_rxd.data_ptr(), M, d_el.data_ptr())
+        lls.synchronize()
+        t_lls = time.perf_counter() - t0
+        el = d_el.cpu().numpy()
+        hd_l = datagen.hard_decision(el[:, 0] + 1j * el[:, 1], data["bps"])
+        out["lls"] = {"user": w["users"][0], "ms": 1000 * t_lls,
+                      "ser": float(np.mean(hd_l != data["tx_data"][w["users"][0]])),
+                      "apsm_ser_same_user": float(np.mean(hd[0] != tx_data[0]))}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, data, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,142 @@
+/*
+ * apsm_capi.h - C ABI of the MI355X (gfx950) APSM / LLS multi-user detector library
+ * (libapsm_amd.so). Plain pointers and sizes only; no C++ or torch types.
+ *
+ * Every entry point stands in for a piece of the reference library's C++ API
+ * (fraunhoferhhi/libapsm); the citation on each declaration names the reference
+ * function it replaces. The drop-in C++ classes (NomaDetector, ApsmNomaDetector,
+ * LlsNomaDetector, ...) are thin wrappers over these calls.
+ *
+ * Data layouts (the reference's host layouts, cpp/lib/noma/types.cuh:69-100):
+ *   rx  : float [num_antennas][num_samples][2]   complex, antenna-major ([antenna][sample])
+ *   tx  : float [num_users][num_samples][2]      complex pilots, one row per detected user
+ *   est : float [num_users][num_samples][2]      complex soft estimates
+ * The *_dev entry points take the same layouts in device memory and run on the
+ * handle's stream without host synchronisation.
+ *
+ * Errors: every call returns APSM_OK (0) or a negative status; apsm_last_error()
+ * returns the message of the last failure on the calling thread. Handles are not
+ * thread-safe; distinct handles may be used from different threads.
+ */
+#ifndef APSM_CAPI_H
+#define APSM_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define APSM_OK 0
+#define APSM_ERR_ARG -1 /* invalid argument or configuration (reference: std::invalid_argument) */
+#define APSM_ERR_DEVICE -2 /* HIP runtime failure (reference: CUDA_CHECK -> exit) */
+#define APSM_ERR_NUMERIC -3 /* least-squares system not positive definite */
+#define APSM_ERR_UNSUPPORTED -4 /* configuration the device path does not implement */
+
+/* detectVersion values (cpp/lib/apsm/apsm_versions.cuh:75-89). All four select the same
+ * dictionary-streaming kernel here; the names are accepted for drop-in compatibility. */
+#define APSM_DETECT_ORIGINAL 0x10001
+#define APSM_DETECT_OLDFAST 0x10002
+#define APSM_DETECT_SHMEM 0x10006
+#define APSM_DETECT_BALANCED 0x10007
+
+/* Parameter set. Field meaning, units and defaults follow apsm_parameters
+ * (cpp/lib/apsm/apsm_parameters.cuh:68-86) and the factory defaults
+ * (cpp/lib/apsm/apsm_noma_detector.cu:361-372). window_size and dictionary_size count
+ * complex samples; sample_step counts real rows. gaussian_kernel_variance is the
+ * unscaled configuration value: the library applies var = v * N^2 when v != 0
+ * (apsm_noma_detector.cu:79-80) and the pairwise-distance heuristic when v == 0
+ * (apsm_noma_detector.cu:194-218). */
+typedef struct apsm_params
+{
+    float gaussian_kernel_weight;
+    float gaussian_kernel_variance;
+    float eB;
+    float norm_constraint;
+    uint32_t window_size;
+    uint32_t sample_step;
+    uint32_t train_passes;
+    uint32_t dictionary_size;
+    uint8_t start_with_full_window;
+    uint8_t lls_initialization;
+    uint8_t training_shuffle;
+    uint8_t reserved;
+    int32_t detect_version;
+} apsm_params;
+
+typedef struct apsm_handle apsm_handle;
+typedef struct lls_handle lls_handle;
+
+/* Fills *p with the reference factory defaults (apsm_noma_detector.cu:361-372). */
+void apsm_params_default(apsm_params* p);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* apsm_last_error(void);
+
+/* Replaces ApsmNomaDetector<version>::ApsmNomaDetector(par, numAntennas)
+ * (cpp/lib/apsm/apsm_noma_detector.cu:72-81). num_users > 1 trains that many detectors
+ * on the same received signal in one handle (one pilot row per user in tx); they share
+ * the Gaussian dictionary, which is made of the received pilots. */
+int apsm_create(const apsm_params* p, uint32_t num_antennas, uint32_t num_users, apsm_handle** out);
+
+/* Replaces ApsmNomaDetector::~ApsmNomaDetector (apsm_noma_detector.cu:83-87). */
+void apsm_destroy(apsm_handle* h);
+
+/* Replaces ApsmNomaDetector::resetState (apsm_noma_detector.cu:89-93). */
+int apsm_reset(apsm_handle* h);
+
+/* Replaces ApsmNomaDetector::train (apsm_noma_detector.cu:96-128 and the host loop
+ * apsm_train.cu:80-166). Host rx [N][T][2], tx [U][T][2]. *ms (may be NULL) receives the
+ * device time of the train loop, like the reference's event timer (apsm_train.cu:99,163-165).
+ * A later call without apsm_reset continues the same pilot stream. */
+int apsm_train(apsm_handle* h, const float* rx, const float* tx, uint32_t T, float* ms);
+
+/* Replaces ApsmNomaDetector::detect (apsm_noma_detector.cu:131-143). Host rx [N][M][2],
+ * est [U][M][2]. *ms receives the detect kernel time (apsm_detect.cu:972-974). */
+int apsm_detect(apsm_handle* h, const float* rx, uint32_t M, float* est, float* ms);
+
+/* Device-memory variants of apsm_train / apsm_detect (no counterpart in the reference,
+ * which always uploads from host vectors, cuda_matrix_host.cu:105-126). Same layouts,
+ * device pointers, enqueued on the handle's stream; no host synchronisation. */
+int apsm_train_dev(apsm_handle* h, const float* d_rx, const float* d_tx, uint32_t T);
+int apsm_detect_dev(apsm_handle* h, const float* d_rx, uint32_t M, float* d_est);
+
+/* Device time (HIP events recorded on the handle's stream) of the most recent train
+ * sequencer launch and the most recent detect launch, in ms; waits for those events.
+ * Stands in for the event timer the reference returns from train/detect
+ * (cpp/lib/cuda/cuda_eventtimer.cuh:73-133); covers the *_dev calls as well. */
+int apsm_kernel_times(apsm_handle* h, float* train_ms, float* detect_ms);
+
+/* Waits for all work enqueued on the handle's stream. */
+int apsm_synchronize(apsm_handle* h);
+
+/* The handle's stream as an opaque hipStream_t. */
+void* apsm_stream(apsm_handle* h);
+
+/* Training state readback, in the spirit of getTrainingState()
+ * (apsm_noma_detector.cuh:93): stream length in real rows, dictionary size in complex
+ * entries, the scaled kernel variance, theta [U][2N], and the real-row coefficients of the
+ * dictionary [U][2*dict] in stream order (oldest first). Any output pointer may be NULL. */
+int apsm_get_state(apsm_handle* h, uint64_t* used_rows, uint32_t* dict_entries, float* variance,
+                   float* theta, float* coeffs, uint32_t coeffs_cap);
+
+/* LLS detector: replaces LlsNomaDetector (cpp/lib/noma/lls_detector.cu:97-218).
+ * Normal equations accumulated on the device in FP32, solved by an FP64 Cholesky on the
+ * device; detect is est = A w. lls_get_weights replaces getWeightMat() (2N floats). */
+int lls_create(uint32_t num_antennas, lls_handle** out);
+void lls_destroy(lls_handle* h);
+int lls_train(lls_handle* h, const float* rx, const float* tx, uint32_t T, float* ms);
+int lls_detect(lls_handle* h, const float* rx, uint32_t M, float* est, float* ms);
+int lls_train_dev(lls_handle* h, const float* d_rx, const float* d_tx, uint32_t T);
+int lls_detect_dev(lls_handle* h, const float* d_rx, uint32_t M, float* d_est);
+int lls_get_weights(lls_handle* h, float* w);
+int lls_synchronize(lls_handle* h);
+
+/* Number of visible HIP devices (0 when no GPU is present). Never fails. */
+int apsm_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APSM_CAPI_H */

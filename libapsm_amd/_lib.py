@@ -1,0 +1,91 @@
+"""ctypes loader for libapsm_amd.so (the C ABI in include/apsm_capi.h)."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_HERE, "libapsm_amd.so")
+
+APSM_OK = 0
+APSM_ERR_ARG = -1
+APSM_ERR_DEVICE = -2
+APSM_ERR_NUMERIC = -3
+APSM_ERR_UNSUPPORTED = -4
+
+
+class ApsmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class ApsmParamsC(ctypes.Structure):
+    _fields_ = [
+        ("gaussian_kernel_weight", ctypes.c_float),
+        ("gaussian_kernel_variance", ctypes.c_float),
+        ("eB", ctypes.c_float),
+        ("norm_constraint", ctypes.c_float),
+        ("window_size", ctypes.c_uint32),
+        ("sample_step", ctypes.c_uint32),
+        ("train_passes", ctypes.c_uint32),
+        ("dictionary_size", ctypes.c_uint32),
+        ("start_with_full_window", ctypes.c_uint8),
+        ("lls_initialization", ctypes.c_uint8),
+        ("training_shuffle", ctypes.c_uint8),
+        ("reserved", ctypes.c_uint8),
+        ("detect_version", ctypes.c_int32),
+    ]
+
+
+# Every symbol include/apsm_capi.h declares, with (restype, argtypes).
+_P = ctypes.POINTER
+_vp = ctypes.c_void_p
+_f32p = _P(ctypes.c_float)
+_u32 = ctypes.c_uint32
+SIGNATURES = {
+    "apsm_params_default": (None, [_P(ApsmParamsC)]),
+    "apsm_last_error": (ctypes.c_char_p, []),
+    "apsm_device_count": (ctypes.c_int, []),
+    "apsm_create": (ctypes.c_int, [_P(ApsmParamsC), _u32, _u32, _P(_vp)]),
+    "apsm_destroy": (None, [_vp]),
+    "apsm_reset": (ctypes.c_int, [_vp]),
+    "apsm_train": (ctypes.c_int, [_vp, _vp, _vp, _u32, _f32p]),
+    "apsm_detect": (ctypes.c_int, [_vp, _vp, _u32, _vp, _f32p]),
+    "apsm_train_dev": (ctypes.c_int, [_vp, _vp, _vp, _u32]),
+    "apsm_detect_dev": (ctypes.c_int, [_vp, _vp, _u32, _vp]),
+    "apsm_kernel_times": (ctypes.c_int, [_vp, _f32p, _f32p]),
+    "apsm_synchronize": (ctypes.c_int, [_vp]),
+    "apsm_stream": (_vp, [_vp]),
+    "apsm_get_state": (ctypes.c_int, [_vp, _P(ctypes.c_uint64), _P(ctypes.c_uint32), _f32p, _vp, _vp, _u32]),
+    "lls_create": (ctypes.c_int, [_u32, _P(_vp)]),
+    "lls_destroy": (None, [_vp]),
+    "lls_train": (ctypes.c_int, [_vp, _vp, _vp, _u32, _f32p]),
+    "lls_detect": (ctypes.c_int, [_vp, _vp, _u32, _vp, _f32p]),
+    "lls_train_dev": (ctypes.c_int, [_vp, _vp, _vp, _u32]),
+    "lls_detect_dev": (ctypes.c_int, [_vp, _vp, _u32, _vp]),
+    "lls_get_weights": (ctypes.c_int, [_vp, _vp]),
+    "lls_synchronize": (ctypes.c_int, [_vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libapsm_amd.so; raises if it has not been built (no CPU fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise ImportError(
+                f"{library_path} is missing: build it with `make -C libapsm_amd/csrc` "
+                "(or __graft_entry__.build())")
+        L = ctypes.CDLL(library_path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != APSM_OK:
+        raise ApsmError(rc, lib().apsm_last_error().decode())

@@ -1,0 +1,826 @@
+// Host side of libapsm_amd: the C ABI declared in include/apsm_capi.h.
+//
+// One apsm_handle owns its device state (stream buffer, window state, linear weights), one
+// HIP stream and two events. Host entry points upload into handle-owned device buffers and
+// call the *_dev entry points; those enqueue kernels on the handle's stream.
+
+#include "apsm_capi.h"
+#include "apsm_device.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_lastError;
+
+struct ApsmError
+{
+    int code;
+    std::string msg;
+};
+
+#define HIP_OR_THROW( expr )                                                                                                  \
+    do                                                                                                                        \
+    {                                                                                                                         \
+        hipError_t e_ = ( expr );                                                                                             \
+        if ( e_ != hipSuccess )                                                                                               \
+            throw ApsmError { APSM_ERR_DEVICE, std::string( #expr ) + ": " + hipGetErrorString( e_ ) };                      \
+    } while ( 0 )
+
+#define ARG_CHECK( cond, msg )                                                                                                \
+    do                                                                                                                        \
+    {                                                                                                                         \
+        if ( !( cond ) )                                                                                                      \
+            throw ApsmError { APSM_ERR_ARG, msg };                                                                            \
+    } while ( 0 )
+
+template <typename F>
+int guarded( F&& f )
+{
+    try
+    {
+        f();
+        return APSM_OK;
+    }
+    catch ( const ApsmError& e )
+    {
+        g_lastError = e.msg;
+        return e.code;
+    }
+    catch ( const std::exception& e )
+    {
+        g_lastError = e.what();
+        return APSM_ERR_ARG;
+    }
+}
+
+// Device buffer that grows on demand (contents are not preserved on growth).
+template <typename T>
+struct DevBuf
+{
+    T* ptr = nullptr;
+    size_t cap = 0;
+    void ensure( size_t n )
+    {
+        if ( n <= cap )
+            return;
+        if ( ptr )
+            HIP_OR_THROW( hipFree( ptr ) );
+        ptr = nullptr;
+        cap = 0;
+        if ( n == 0 )
+            return;
+        HIP_OR_THROW( hipMalloc( &ptr, n * sizeof( T ) ) );
+        cap = n;
+    }
+    void release()
+    {
+        if ( ptr )
+            (void)hipFree( ptr );
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+struct StreamBuf
+{
+    DevBuf<float> Z, Zn, Y, C;
+    int64_t capK = 0;
+    void ensure( int64_t K, int N, int U )
+    {
+        if ( K <= capK )
+            return;
+        int64_t k2 = std::max<int64_t>( K, capK + capK / 2 );
+        Z.ensure( (size_t)k2 * 2 * N );
+        Zn.ensure( (size_t)k2 );
+        Y.ensure( (size_t)k2 * 2 * U );
+        C.ensure( (size_t)k2 * 2 * U );
+        capK = k2;
+    }
+    void release()
+    {
+        Z.release();
+        Zn.release();
+        Y.release();
+        C.release();
+        capK = 0;
+    }
+};
+
+constexpr float kLog2e = 1.4426950408889634f;
+
+// Pairwise-distance heuristic for the Gaussian variance (cpp/lib/apsm/apsm_noma_detector.cu:194-218),
+// same FP32 arithmetic so the percentile index matches.
+float varianceHeuristic( const float* rx, int N, int64_t T, const std::vector<int32_t>* perm )
+{
+    std::vector<float> nd;
+    nd.reserve( (size_t)( T * ( T - 1 ) / 2 ) );
+    for ( int64_t i = 0; i < T; i++ )
+        for ( int64_t j = i + 1; j < T; j++ )
+        {
+            int64_t pi = perm ? ( *perm )[ i ] : i, pj = perm ? ( *perm )[ j ] : j;
+            float dist = 0;
+            for ( int n = 0; n < N; n++ )
+            {
+                std::complex<float> a( rx[ ( n * T + pi ) * 2 ], rx[ ( n * T + pi ) * 2 + 1 ] );
+                std::complex<float> b( rx[ ( n * T + pj ) * 2 ], rx[ ( n * T + pj ) * 2 + 1 ] );
+                dist += std::abs( a - b );
+            }
+            nd.push_back( std::sqrt( dist ) );
+        }
+    if ( nd.empty() )
+        throw ApsmError { APSM_ERR_ARG, "variance heuristic needs at least two pilots" };
+    std::sort( nd.begin(), nd.end() );
+    const float p = 0.7f, q = 5.f;
+    float v = nd[ (size_t)( nd.size() * p ) ] * std::pow( (float)T, -1.f / q );
+    return v * v;
+}
+
+// Permutation of the reference shuffle (cpp/lib/noma/noma_detector.cu:59-74).
+std::vector<int32_t> shufflePermutation( int64_t T )
+{
+    std::vector<int32_t> perm( T );
+    for ( int64_t i = 0; i < T; i++ )
+        perm[ i ] = (int32_t)i;
+    std::seed_seq seed { 1, 2, 3, 4, 5, 6, 7, 8 };
+    std::mt19937 rng( seed );
+    std::shuffle( perm.begin(), perm.end(), rng );
+    return perm;
+}
+
+int gridFor( int64_t n, int threads ) { return (int)( ( n + threads - 1 ) / threads ); }
+
+} // namespace
+
+// ---------------------------------------------------------------------------------------
+// LLS
+// ---------------------------------------------------------------------------------------
+struct lls_handle
+{
+    int N = 0, L = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf<float> w, part, rx, tx, est;
+    DevBuf<double> gram;
+    DevBuf<int> status;
+    bool trained = false;
+};
+
+namespace {
+
+void llsTrainDev( lls_handle* h, const float* d_rx, const float* d_tx, int64_t T )
+{
+    ARG_CHECK( T > 0, "LLS train needs at least one pilot" );
+    const int L = h->L;
+    const int E = L * ( L + 1 );
+    const int64_t chunk = std::max<int64_t>( 1, (int64_t)( apsm::kLlsLdsBudget / ( 2 * ( L + 1 ) * sizeof( float ) ) ) );
+    const int nPart = (int)( ( T + chunk - 1 ) / chunk );
+    h->part.ensure( (size_t)nPart * E );
+    h->gram.ensure( (size_t)E );
+    const size_t lds = 2 * (size_t)chunk * ( L + 1 ) * sizeof( float );
+    hipLaunchKernelGGL( apsm::lls_normal_partial, dim3( nPart ), dim3( apsm::kLlsThreads ), lds, h->stream, d_rx, d_tx, h->N,
+                        T, chunk, h->part.ptr );
+    HIP_OR_THROW( hipGetLastError() );
+    hipLaunchKernelGGL( apsm::lls_reduce_partials, dim3( gridFor( E, 256 ) ), dim3( 256 ), 0, h->stream, h->part.ptr, nPart, E,
+                        h->gram.ptr );
+    HIP_OR_THROW( hipGetLastError() );
+    const size_t ldsSolve = (size_t)E * sizeof( double );
+    if ( ldsSolve > 64 * 1024 )
+        HIP_OR_THROW( hipFuncSetAttribute( (const void*)apsm::lls_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsSolve ) );
+    hipLaunchKernelGGL( apsm::lls_solve, dim3( 1 ), dim3( apsm::kSolveThreads ), ldsSolve, h->stream, h->gram.ptr, L, h->w.ptr,
+                        h->status.ptr );
+    HIP_OR_THROW( hipGetLastError() );
+}
+
+void llsCheckStatus( lls_handle* h )
+{
+    int st = 0;
+    HIP_OR_THROW( hipMemcpyAsync( &st, h->status.ptr, sizeof( int ), hipMemcpyDeviceToHost, h->stream ) );
+    HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+    if ( st != 0 )
+        throw ApsmError { APSM_ERR_NUMERIC, "least-squares normal matrix is not positive definite" };
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------------------
+// APSM handle
+// ---------------------------------------------------------------------------------------
+struct apsm_handle
+{
+    apsm_params p {};
+    int N = 0, L = 0, U = 0, w2 = 0;
+    int64_t H = 0, Hc = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr; // train sequencer
+    hipEvent_t ev2 = nullptr, ev3 = nullptr; // detect
+    bool trainTimed = false, detectTimed = false;
+    float var = 0.f;
+    bool varSet = false;
+    bool anyTrain = false;
+    uint64_t used = 0; // absolute real rows entered into the pilot stream
+    StreamBuf buf[ 2 ];
+    int cur = 0;
+    int64_t bufBaseC = 0; // absolute complex index of entry 0 of buf[cur]
+    int64_t bufK = 0; // complex entries staged in buf[cur]
+    DevBuf<float> F, Gram, thOld, theta;
+    DevBuf<float> rx, tx, est; // staging for the host entry points
+    DevBuf<int32_t> perm;
+    lls_handle* lls = nullptr;
+};
+
+namespace {
+
+void resetDeviceState( apsm_handle* h )
+{
+    const size_t nF = (size_t)h->U * h->w2, nG = (size_t)h->U * h->w2 * h->w2, nT = (size_t)h->U * h->L;
+    HIP_OR_THROW( hipMemsetAsync( h->F.ptr, 0, nF * sizeof( float ), h->stream ) );
+    HIP_OR_THROW( hipMemsetAsync( h->Gram.ptr, 0, nG * sizeof( float ), h->stream ) );
+    HIP_OR_THROW( hipMemsetAsync( h->thOld.ptr, 0, nT * sizeof( float ), h->stream ) );
+    HIP_OR_THROW( hipMemsetAsync( h->theta.ptr, 0, nT * sizeof( float ), h->stream ) );
+    h->used = 0;
+    h->bufBaseC = 0;
+    h->bufK = 0;
+    h->anyTrain = false;
+}
+
+void validateParams( const apsm_params* p, uint32_t N, uint32_t U )
+{
+    ARG_CHECK( p != nullptr, "null parameter set" );
+    ARG_CHECK( N >= 1 && N <= 64, "numAntennas must be in [1, 64]" );
+    ARG_CHECK( U >= 1 && U <= 1024, "numUsers must be in [1, 1024]" );
+    ARG_CHECK( p->dictionary_size >= 1, "dictionarySize must be positive" );
+    ARG_CHECK( p->window_size >= 1 && p->window_size <= p->dictionary_size, "windowSize must be in [1, dictionarySize]" );
+    ARG_CHECK( 2 * p->window_size <= 1024, "windowSize must be at most 512" );
+    ARG_CHECK( p->sample_step >= 2 && p->sample_step % 2 == 0, "sampleStep must be a positive even number" );
+    ARG_CHECK( p->train_passes >= 1, "trainPasses must be at least 1" );
+    ARG_CHECK( p->gaussian_kernel_weight >= 0.f && p->gaussian_kernel_weight < 1.f, "gaussianKernelWeight must be in [0, 1)" );
+    ARG_CHECK( p->gaussian_kernel_variance >= 0.f, "gaussianKernelVariance must be non-negative" );
+    ARG_CHECK( p->eB >= 0.f, "eB must be non-negative" );
+    if ( p->norm_constraint > 0.f )
+        throw ApsmError { APSM_ERR_UNSUPPORTED, "normConstraint > 0 is not implemented on the device path" };
+    switch ( p->detect_version )
+    {
+    case APSM_DETECT_ORIGINAL:
+    case APSM_DETECT_OLDFAST:
+    case APSM_DETECT_SHMEM:
+    case APSM_DETECT_BALANCED:
+        break;
+    default:
+        throw ApsmError { APSM_ERR_ARG, "unknown detect version" };
+    }
+}
+
+void trainDev( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, const float* hostRx )
+{
+    const apsm_params& p = h->p;
+    ARG_CHECK( T > 0, "train needs at least one pilot" );
+    ARG_CHECK( p.train_passes == 1 || h->used == 0, "trainPasses > 1 requires a fresh state (resetState)" );
+    const int N = h->N, L = h->L, U = h->U;
+
+    std::vector<int32_t> perm;
+    if ( p.training_shuffle )
+        perm = shufflePermutation( T );
+
+    if ( !h->varSet )
+    {
+        if ( p.gaussian_kernel_variance != 0.f )
+            h->var = p.gaussian_kernel_variance * (float)( N * N );
+        else
+        {
+            std::vector<float> tmp;
+            const float* src = hostRx;
+            if ( !src )
+            {
+                tmp.resize( (size_t)N * T * 2 );
+                HIP_OR_THROW( hipMemcpyAsync( tmp.data(), d_rx, tmp.size() * sizeof( float ), hipMemcpyDeviceToHost, h->stream ) );
+                HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+                src = tmp.data();
+            }
+            h->var = varianceHeuristic( src, N, T, p.training_shuffle ? &perm : nullptr );
+        }
+        h->varSet = true;
+    }
+    const float wG = p.gaussian_kernel_weight, wL = 1.f - wG;
+
+    if ( p.lls_initialization && !h->anyTrain )
+    {
+        // theta_0 = w_LLS / wL per user (apsm_noma_detector.cu:321-346)
+        if ( !h->lls )
+        {
+            lls_handle* l = nullptr;
+            if ( lls_create( N, &l ) != APSM_OK )
+                throw ApsmError { APSM_ERR_DEVICE, g_lastError };
+            h->lls = l;
+        }
+        for ( int u = 0; u < U; u++ )
+        {
+            HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+            llsTrainDev( h->lls, d_rx, d_tx + (size_t)u * T * 2, T );
+            llsCheckStatus( h->lls );
+            hipLaunchKernelGGL( apsm::apsm_lls_init_theta, dim3( 1 ), dim3( 128 ), 0, h->lls->stream, h->lls->w.ptr,
+                                h->thOld.ptr + (size_t)u * L, L, wL );
+            HIP_OR_THROW( hipGetLastError() );
+            HIP_OR_THROW( hipStreamSynchronize( h->lls->stream ) );
+        }
+    }
+    h->anyTrain = true;
+
+    // New stream buffer: the dictionary rows still live (at most Dc complex entries) + this call.
+    const int64_t usedC = (int64_t)( h->used / 2 );
+    const int64_t nOld = std::min<int64_t>( usedC, h->Hc );
+    const int nxt = h->cur ^ 1;
+    StreamBuf& nb = h->buf[ nxt ];
+    const int64_t K = nOld + T;
+    nb.ensure( K, N, U );
+    const int64_t ld = 2 * nb.capK;
+    if ( nOld > 0 )
+    {
+        StreamBuf& ob = h->buf[ h->cur ];
+        const int64_t off = usedC - nOld - h->bufBaseC;
+        const int64_t ldOld = 2 * ob.capK;
+        HIP_OR_THROW( hipMemcpyAsync( nb.Z.ptr, ob.Z.ptr + off * L, (size_t)nOld * L * sizeof( float ), hipMemcpyDeviceToDevice,
+                                      h->stream ) );
+        HIP_OR_THROW( hipMemcpyAsync( nb.Zn.ptr, ob.Zn.ptr + off, (size_t)nOld * sizeof( float ), hipMemcpyDeviceToDevice, h->stream ) );
+        HIP_OR_THROW( hipMemcpy2DAsync( nb.Y.ptr, ld * sizeof( float ), ob.Y.ptr + 2 * off, ldOld * sizeof( float ),
+                                        2 * nOld * sizeof( float ), U, hipMemcpyDeviceToDevice, h->stream ) );
+        HIP_OR_THROW( hipMemcpy2DAsync( nb.C.ptr, ld * sizeof( float ), ob.C.ptr + 2 * off, ldOld * sizeof( float ),
+                                        2 * nOld * sizeof( float ), U, hipMemcpyDeviceToDevice, h->stream ) );
+    }
+    const int32_t* dPerm = nullptr;
+    if ( p.training_shuffle )
+    {
+        h->perm.ensure( (size_t)T );
+        HIP_OR_THROW( hipMemcpyAsync( h->perm.ptr, perm.data(), T * sizeof( int32_t ), hipMemcpyHostToDevice, h->stream ) );
+        dPerm = h->perm.ptr;
+    }
+    hipLaunchKernelGGL( apsm::stage_pilots, dim3( gridFor( T, 256 ) ), dim3( 256 ), 0, h->stream, d_rx, d_tx, dPerm, N, U, T,
+                        nb.Z.ptr + nOld * L, nb.Zn.ptr + nOld, nb.Y.ptr + 2 * nOld, nb.C.ptr + 2 * nOld, ld );
+    HIP_OR_THROW( hipGetLastError() );
+
+    apsm::SeqArgs a {};
+    a.N = N;
+    a.L = L;
+    a.U = U;
+    a.w2 = h->w2;
+    a.q = (int)p.sample_step;
+    a.passes = (int)p.train_passes;
+    a.fullStart = p.start_with_full_window ? 1 : 0;
+    a.H = h->H;
+    a.usedOld = (int64_t)h->used;
+    a.R2 = 2 * T;
+    a.rowBase = 2 * ( usedC - nOld );
+    a.ldYC = ld;
+    a.wL = wL;
+    a.wG = wG;
+    a.g2 = 0.5f / h->var * kLog2e;
+    a.eB = p.eB;
+
+    const size_t lds = sizeof( float )
+        * ( 2 * (size_t)h->w2 + L + apsm::kNewMax * L + apsm::kNewMax + apsm::kSeqWaves * 2 * apsm::kNewMax
+            + apsm::kSeqWaves * apsm::kWave );
+    HIP_OR_THROW( hipEventRecord( h->ev0, h->stream ) );
+    hipLaunchKernelGGL( apsm::apsm_train_seq, dim3( U ), dim3( apsm::kSeqThreads ), lds, h->stream, nb.Z.ptr, nb.Zn.ptr, nb.Y.ptr,
+                        nb.C.ptr, h->F.ptr, h->Gram.ptr, h->thOld.ptr, a );
+    HIP_OR_THROW( hipGetLastError() );
+    HIP_OR_THROW( hipEventRecord( h->ev1, h->stream ) );
+    h->trainTimed = true;
+
+    // stream bookkeeping, same schedule as the kernel
+    const int64_t first = ( p.start_with_full_window && h->used == 0 ) ? h->w2 : p.sample_step;
+    const int64_t R2 = 2 * T;
+    if ( R2 >= first )
+        h->used += (uint64_t)( first + ( ( R2 - first ) / p.sample_step ) * p.sample_step );
+    h->cur = nxt;
+    h->bufBaseC = usedC - nOld;
+    h->bufK = K;
+
+    const int64_t newUsedC = (int64_t)( h->used / 2 );
+    const int64_t lo = std::max<int64_t>( 0, newUsedC - h->Hc );
+    hipLaunchKernelGGL( apsm::apsm_theta_full, dim3( U ), dim3( 256 ), 0, h->stream, nb.Z.ptr, nb.C.ptr, h->thOld.ptr,
+                        h->theta.ptr, N, lo - h->bufBaseC, newUsedC - h->bufBaseC, ld );
+    HIP_OR_THROW( hipGetLastError() );
+}
+
+template <int NMAX, int UMAX>
+void launchDetect( apsm_handle* h, const float* d_rx, int64_t M, float* d_est, const float* Z, const float* Zn, const float* C,
+                   int64_t ldC, const float* theta, int64_t G, int U )
+{
+    const float wG = h->p.gaussian_kernel_weight, wL = 1.f - wG;
+    const float g2 = 0.5f / h->var * kLog2e;
+    hipLaunchKernelGGL( ( apsm::apsm_detect_kernel<NMAX, UMAX> ), dim3( gridFor( M, apsm::kDetThreads ) ), dim3( apsm::kDetThreads ),
+                        0, h->stream, d_rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, d_est, h->N );
+    HIP_OR_THROW( hipGetLastError() );
+}
+
+template <int NMAX>
+void launchDetectU( apsm_handle* h, const float* d_rx, int64_t M, float* d_est, const float* Z, const float* Zn, const float* C,
+                    int64_t ldC, const float* theta, int64_t G, int U )
+{
+    if ( U <= 1 )
+        launchDetect<NMAX, 1>( h, d_rx, M, d_est, Z, Zn, C, ldC, theta, G, U );
+    else if ( U <= 2 )
+        launchDetect<NMAX, 2>( h, d_rx, M, d_est, Z, Zn, C, ldC, theta, G, U );
+    else if ( U <= 4 )
+        launchDetect<NMAX, 4>( h, d_rx, M, d_est, Z, Zn, C, ldC, theta, G, U );
+    else if ( U <= 8 )
+        launchDetect<NMAX, 8>( h, d_rx, M, d_est, Z, Zn, C, ldC, theta, G, U );
+    else
+        launchDetect<NMAX, 16>( h, d_rx, M, d_est, Z, Zn, C, ldC, theta, G, U );
+}
+
+void detectDev( apsm_handle* h, const float* d_rx, int64_t M, float* d_est )
+{
+    ARG_CHECK( M > 0, "detect needs at least one sample" );
+    if ( !h->varSet )
+        h->var = h->p.gaussian_kernel_variance != 0.f ? h->p.gaussian_kernel_variance * (float)( h->N * h->N ) : 1.f;
+    const int64_t usedC = (int64_t)( h->used / 2 );
+    const int64_t lo = std::max<int64_t>( 0, usedC - h->Hc );
+    const int64_t G = usedC - lo;
+    StreamBuf& b = h->buf[ h->cur ];
+    const int64_t off = lo - h->bufBaseC;
+    const int64_t ld = 2 * b.capK;
+    const float* Z = G > 0 ? b.Z.ptr + off * h->L : nullptr;
+    const float* Zn = G > 0 ? b.Zn.ptr + off : nullptr;
+    const float* C = G > 0 ? b.C.ptr + 2 * off : nullptr;
+    HIP_OR_THROW( hipEventRecord( h->ev2, h->stream ) );
+    for ( int u0 = 0; u0 < h->U; u0 += 16 )
+    {
+        const int Ug = std::min( 16, h->U - u0 );
+        const float* Cg = C ? C + (int64_t)u0 * ld : nullptr;
+        const float* th = h->theta.ptr + (size_t)u0 * h->L;
+        float* est = d_est + (size_t)u0 * M * 2;
+        if ( h->N <= 4 )
+            launchDetectU<4>( h, d_rx, M, est, Z, Zn, Cg, ld, th, G, Ug );
+        else if ( h->N <= 8 )
+            launchDetectU<8>( h, d_rx, M, est, Z, Zn, Cg, ld, th, G, Ug );
+        else if ( h->N <= 16 )
+            launchDetectU<16>( h, d_rx, M, est, Z, Zn, Cg, ld, th, G, Ug );
+        else if ( h->N <= 32 )
+            launchDetectU<32>( h, d_rx, M, est, Z, Zn, Cg, ld, th, G, Ug );
+        else
+            launchDetectU<64>( h, d_rx, M, est, Z, Zn, Cg, ld, th, G, Ug );
+    }
+    HIP_OR_THROW( hipEventRecord( h->ev3, h->stream ) );
+    h->detectTimed = true;
+}
+
+float elapsedMs( hipEvent_t a, hipEvent_t b )
+{
+    float ms = 0.f;
+    HIP_OR_THROW( hipEventSynchronize( b ) );
+    HIP_OR_THROW( hipEventElapsedTime( &ms, a, b ) );
+    return ms;
+}
+
+} // namespace
+
+extern "C" {
+
+void apsm_params_default( apsm_params* p )
+{
+    if ( !p )
+        return;
+    std::memset( p, 0, sizeof( *p ) );
+    p->gaussian_kernel_weight = 0.5f;
+    p->gaussian_kernel_variance = 0.05f;
+    p->eB = 0.001f;
+    p->norm_constraint = 0.f;
+    p->window_size = 20;
+    p->sample_step = 2;
+    p->train_passes = 1;
+    p->dictionary_size = 685;
+    p->start_with_full_window = 0;
+    p->lls_initialization = 0;
+    p->training_shuffle = 0;
+    p->detect_version = APSM_DETECT_BALANCED;
+}
+
+const char* apsm_last_error( void ) { return g_lastError.c_str(); }
+
+int apsm_device_count( void )
+{
+    int n = 0;
+    if ( hipGetDeviceCount( &n ) != hipSuccess )
+        return 0;
+    return n;
+}
+
+int apsm_create( const apsm_params* p, uint32_t num_antennas, uint32_t num_users, apsm_handle** out )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( out != nullptr, "null output handle" );
+        *out = nullptr;
+        validateParams( p, num_antennas, num_users );
+        apsm_handle* h = new apsm_handle();
+        try
+        {
+            h->p = *p;
+            h->N = (int)num_antennas;
+            h->L = 2 * h->N;
+            h->U = (int)num_users;
+            h->w2 = 2 * (int)p->window_size;
+            h->Hc = p->dictionary_size;
+            h->H = 2 * h->Hc;
+            HIP_OR_THROW( hipStreamCreateWithFlags( &h->stream, hipStreamNonBlocking ) );
+            HIP_OR_THROW( hipEventCreate( &h->ev0 ) );
+            HIP_OR_THROW( hipEventCreate( &h->ev1 ) );
+            HIP_OR_THROW( hipEventCreate( &h->ev2 ) );
+            HIP_OR_THROW( hipEventCreate( &h->ev3 ) );
+            h->F.ensure( (size_t)h->U * h->w2 );
+            h->Gram.ensure( (size_t)h->U * h->w2 * h->w2 );
+            h->thOld.ensure( (size_t)h->U * h->L );
+            h->theta.ensure( (size_t)h->U * h->L );
+            resetDeviceState( h );
+            HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+        }
+        catch ( ... )
+        {
+            apsm_destroy( h );
+            throw;
+        }
+        *out = h;
+    } );
+}
+
+void apsm_destroy( apsm_handle* h )
+{
+    if ( !h )
+        return;
+    if ( h->stream )
+        (void)hipStreamSynchronize( h->stream );
+    for ( auto& b : h->buf )
+        b.release();
+    h->F.release();
+    h->Gram.release();
+    h->thOld.release();
+    h->theta.release();
+    h->rx.release();
+    h->tx.release();
+    h->est.release();
+    h->perm.release();
+    if ( h->lls )
+        lls_destroy( h->lls );
+    if ( h->ev0 )
+        (void)hipEventDestroy( h->ev0 );
+    if ( h->ev1 )
+        (void)hipEventDestroy( h->ev1 );
+    if ( h->ev2 )
+        (void)hipEventDestroy( h->ev2 );
+    if ( h->ev3 )
+        (void)hipEventDestroy( h->ev3 );
+    if ( h->stream )
+        (void)hipStreamDestroy( h->stream );
+    delete h;
+}
+
+int apsm_kernel_times( apsm_handle* h, float* train_ms, float* detect_ms )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h != nullptr, "null handle" );
+        if ( train_ms )
+            *train_ms = h->trainTimed ? elapsedMs( h->ev0, h->ev1 ) : 0.f;
+        if ( detect_ms )
+            *detect_ms = h->detectTimed ? elapsedMs( h->ev2, h->ev3 ) : 0.f;
+    } );
+}
+
+int apsm_reset( apsm_handle* h )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h != nullptr, "null handle" );
+        resetDeviceState( h );
+        HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+    } );
+}
+
+int apsm_train( apsm_handle* h, const float* rx, const float* tx, uint32_t T, float* ms )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && rx && tx, "null argument" );
+        const size_t nrx = (size_t)h->N * T * 2, ntx = (size_t)h->U * T * 2;
+        h->rx.ensure( nrx );
+        h->tx.ensure( ntx );
+        HIP_OR_THROW( hipMemcpyAsync( h->rx.ptr, rx, nrx * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
+        HIP_OR_THROW( hipMemcpyAsync( h->tx.ptr, tx, ntx * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
+        trainDev( h, h->rx.ptr, h->tx.ptr, T, rx );
+        float t = elapsedMs( h->ev0, h->ev1 );
+        HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+        if ( ms )
+            *ms = t;
+    } );
+}
+
+int apsm_train_dev( apsm_handle* h, const float* d_rx, const float* d_tx, uint32_t T )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && d_rx && d_tx, "null argument" );
+        trainDev( h, d_rx, d_tx, T, nullptr );
+    } );
+}
+
+int apsm_detect( apsm_handle* h, const float* rx, uint32_t M, float* est, float* ms )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && rx && est, "null argument" );
+        const size_t nrx = (size_t)h->N * M * 2, nest = (size_t)h->U * M * 2;
+        h->rx.ensure( nrx );
+        h->est.ensure( nest );
+        HIP_OR_THROW( hipMemcpyAsync( h->rx.ptr, rx, nrx * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
+        detectDev( h, h->rx.ptr, M, h->est.ptr );
+        HIP_OR_THROW( hipMemcpyAsync( est, h->est.ptr, nest * sizeof( float ), hipMemcpyDeviceToHost, h->stream ) );
+        float t = elapsedMs( h->ev2, h->ev3 );
+        HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+        if ( ms )
+            *ms = t;
+    } );
+}
+
+int apsm_detect_dev( apsm_handle* h, const float* d_rx, uint32_t M, float* d_est )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && d_rx && d_est, "null argument" );
+        detectDev( h, d_rx, M, d_est );
+    } );
+}
+
+int apsm_synchronize( apsm_handle* h )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h != nullptr, "null handle" );
+        HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+    } );
+}
+
+void* apsm_stream( apsm_handle* h ) { return h ? (void*)h->stream : nullptr; }
+
+int apsm_get_state( apsm_handle* h, uint64_t* used_rows, uint32_t* dict_entries, float* variance, float* theta, float* coeffs,
+                    uint32_t coeffs_cap )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h != nullptr, "null handle" );
+        HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+        const int64_t usedC = (int64_t)( h->used / 2 );
+        const int64_t lo = std::max<int64_t>( 0, usedC - h->Hc );
+        const int64_t G = usedC - lo;
+        if ( used_rows )
+            *used_rows = h->used;
+        if ( dict_entries )
+            *dict_entries = (uint32_t)G;
+        if ( variance )
+            *variance = h->var;
+        if ( theta )
+            HIP_OR_THROW( hipMemcpy( theta, h->theta.ptr, (size_t)h->U * h->L * sizeof( float ), hipMemcpyDeviceToHost ) );
+        if ( coeffs && G > 0 )
+        {
+            ARG_CHECK( coeffs_cap >= 2 * G, "coefficient buffer too small" );
+            StreamBuf& b = h->buf[ h->cur ];
+            const int64_t off = lo - h->bufBaseC;
+            HIP_OR_THROW( hipMemcpy2D( coeffs, 2 * G * sizeof( float ), b.C.ptr + 2 * off, 2 * b.capK * sizeof( float ),
+                                       2 * G * sizeof( float ), h->U, hipMemcpyDeviceToHost ) );
+        }
+    } );
+}
+
+// ---- LLS ----
+int lls_create( uint32_t num_antennas, lls_handle** out )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( out != nullptr, "null output handle" );
+        ARG_CHECK( num_antennas >= 1 && num_antennas <= 64, "numAntennas must be in [1, 64]" );
+        *out = nullptr;
+        lls_handle* h = new lls_handle();
+        try
+        {
+            h->N = (int)num_antennas;
+            h->L = 2 * h->N;
+            HIP_OR_THROW( hipStreamCreateWithFlags( &h->stream, hipStreamNonBlocking ) );
+            HIP_OR_THROW( hipEventCreate( &h->ev0 ) );
+            HIP_OR_THROW( hipEventCreate( &h->ev1 ) );
+            h->w.ensure( (size_t)h->L );
+            h->status.ensure( 1 );
+            HIP_OR_THROW( hipMemsetAsync( h->w.ptr, 0, h->L * sizeof( float ), h->stream ) );
+            HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+        }
+        catch ( ... )
+        {
+            lls_destroy( h );
+            throw;
+        }
+        *out = h;
+    } );
+}
+
+void lls_destroy( lls_handle* h )
+{
+    if ( !h )
+        return;
+    if ( h->stream )
+        (void)hipStreamSynchronize( h->stream );
+    h->w.release();
+    h->part.release();
+    h->gram.release();
+    h->rx.release();
+    h->tx.release();
+    h->est.release();
+    h->status.release();
+    if ( h->ev0 )
+        (void)hipEventDestroy( h->ev0 );
+    if ( h->ev1 )
+        (void)hipEventDestroy( h->ev1 );
+    if ( h->stream )
+        (void)hipStreamDestroy( h->stream );
+    delete h;
+}
+
+int lls_train_dev( lls_handle* h, const float* d_rx, const float* d_tx, uint32_t T )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && d_rx && d_tx, "null argument" );
+        llsTrainDev( h, d_rx, d_tx, T );
+        llsCheckStatus( h );
+        h->trained = true;
+    } );
+}
+
+int lls_train( lls_handle* h, const float* rx, const float* tx, uint32_t T, float* ms )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && rx && tx, "null argument" );
+        const size_t nrx = (size_t)h->N * T * 2, ntx = (size_t)T * 2;
+        h->rx.ensure( nrx );
+        h->tx.ensure( ntx );
+        HIP_OR_THROW( hipMemcpyAsync( h->rx.ptr, rx, nrx * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
+        HIP_OR_THROW( hipMemcpyAsync( h->tx.ptr, tx, ntx * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
+        HIP_OR_THROW( hipEventRecord( h->ev0, h->stream ) );
+        llsTrainDev( h, h->rx.ptr, h->tx.ptr, T );
+        HIP_OR_THROW( hipEventRecord( h->ev1, h->stream ) );
+        float t = elapsedMs( h->ev0, h->ev1 );
+        llsCheckStatus( h );
+        h->trained = true;
+        if ( ms )
+            *ms = t;
+    } );
+}
+
+int lls_detect_dev( lls_handle* h, const float* d_rx, uint32_t M, float* d_est )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && d_rx && d_est, "null argument" );
+        ARG_CHECK( M > 0, "detect needs at least one sample" );
+        hipLaunchKernelGGL( apsm::lls_apply, dim3( gridFor( M, 256 ) ), dim3( 256 ), 0, h->stream, d_rx, h->N, (int64_t)M, h->w.ptr,
+                            d_est );
+        HIP_OR_THROW( hipGetLastError() );
+    } );
+}
+
+int lls_detect( lls_handle* h, const float* rx, uint32_t M, float* est, float* ms )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && rx && est, "null argument" );
+        ARG_CHECK( M > 0, "detect needs at least one sample" );
+        const size_t nrx = (size_t)h->N * M * 2, nest = (size_t)M * 2;
+        h->rx.ensure( nrx );
+        h->est.ensure( nest );
+        HIP_OR_THROW( hipMemcpyAsync( h->rx.ptr, rx, nrx * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
+        HIP_OR_THROW( hipEventRecord( h->ev0, h->stream ) );
+        hipLaunchKernelGGL( apsm::lls_apply, dim3( gridFor( M, 256 ) ), dim3( 256 ), 0, h->stream, h->rx.ptr, h->N, (int64_t)M,
+                            h->w.ptr, h->est.ptr );
+        HIP_OR_THROW( hipGetLastError() );
+        HIP_OR_THROW( hipEventRecord( h->ev1, h->stream ) );
+        HIP_OR_THROW( hipMemcpyAsync( est, h->est.ptr, nest * sizeof( float ), hipMemcpyDeviceToHost, h->stream ) );
+        float t = elapsedMs( h->ev0, h->ev1 );
+        HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+        if ( ms )
+            *ms = t;
+    } );
+}
+
+int lls_get_weights( lls_handle* h, float* w )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && w, "null argument" );
+        HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+        HIP_OR_THROW( hipMemcpy( w, h->w.ptr, h->L * sizeof( float ), hipMemcpyDeviceToHost ) );
+    } );
+}
+
+int lls_synchronize( lls_handle* h )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h != nullptr, "null handle" );
+        HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+    } );
+}
+
+} // extern "C"

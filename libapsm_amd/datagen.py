@@ -1,0 +1,113 @@
+"""Seeded synthetic uplink NOMA data (the step before the detector).
+
+The reference ships only a pilot-symbol generator (cpp/cli/common/util.cpp:287-325) and
+no channel model, so the channel here is this project's: per-frame Rayleigh gains
+H[n,u] ~ CN(0,1), optional per-user large-scale gains uniform in dB, AWGN at a given SNR,
+x = H diag(g) s + noise, then per-antenna power normalisation over the pilots.
+
+Symbols use the reference's Gray mapping and unit-energy scales
+(cpp/cli/common/modulation.cpp:112-151, modulation.hpp:157-222).
+"""
+import numpy as np
+
+BITS_PER_SYMBOL = {"BPSK": 1, "QPSK": 2, "QAM16": 4, "QAM64": 6, "QAM256": 8}
+
+
+def mod_scale(bps):
+    """Scale that gives unit average energy (modulation.hpp:157-180)."""
+    if bps in (1, 2):
+        return 1.0 / np.sqrt(bps)
+    return 1.0 / np.sqrt(((2 << bps) - 2) // 3)
+
+
+def _inverse_gray(b):
+    g = b.copy()
+    shift = b >> 1
+    while np.any(shift):
+        g ^= shift
+        shift >>= 1
+    return g
+
+
+def constellation(bps):
+    """Complex point for every bit pattern 0 .. 2^bps - 1 (Gray mapped, unit energy)."""
+    bits = np.arange(1 << bps, dtype=np.int64)
+    s = mod_scale(bps)
+    if bps == 1:
+        return (2.0 * _inverse_gray(bits) - 1.0) * s + 0j
+    h = bps // 2
+    ns = 1 << h
+    re = 2.0 * _inverse_gray(bits >> h) - ns + 1
+    im = 2.0 * _inverse_gray(bits & (ns - 1)) - ns + 1
+    return (re + 1j * im) * s
+
+
+def hard_decision(est, bps):
+    """Nearest constellation point per real dimension (the decision regions of the SER rule)."""
+    s = mod_scale(bps)
+    est = np.asarray(est)
+    if bps == 1:
+        return np.where(est.real >= 0, s, -s) + 0j
+    ns = 1 << (bps // 2)
+    lim = ns - 1
+
+    def q(v):
+        k = np.round((v / s + lim) / 2.0)
+        k = np.clip(k, 0, lim)
+        return (2.0 * k - lim) * s
+
+    return q(est.real) + 1j * q(est.imag)
+
+
+def generate(num_antennas, num_users, num_pilots, num_data, modulation="QPSK", snr_db=10.0,
+             seed=1, large_scale_db=None, normalize=True):
+    """Returns dict(rx_train [N][T], rx_data [N][M], tx_train [U][T], tx_data [U][M]) complex64.
+
+    Pilots come first, then data, from one random stream.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    bps = BITS_PER_SYMBOL[modulation.upper()]
+    S = num_pilots + num_data
+    pts = constellation(bps)
+    sym = pts[rng.integers(0, 1 << bps, size=(num_users, S))]
+    Hc = (rng.standard_normal((num_antennas, num_users)) + 1j * rng.standard_normal((num_antennas, num_users))) / np.sqrt(2)
+    if large_scale_db is not None:
+        lo, hi = large_scale_db
+        g = 10.0 ** (rng.uniform(lo, hi, size=num_users) / 20.0)
+        Hc = Hc * g[None, :]
+    nvar = 10.0 ** (-snr_db / 10.0)
+    noise = (rng.standard_normal((num_antennas, S)) + 1j * rng.standard_normal((num_antennas, S))) * np.sqrt(nvar / 2)
+    x = Hc @ sym + noise
+    if normalize:
+        p = np.sqrt(np.mean(np.abs(x[:, :num_pilots]) ** 2, axis=1, keepdims=True))
+        x = x / p
+    x = x.astype(np.complex64)
+    sym = sym.astype(np.complex64)
+    return dict(rx_train=np.ascontiguousarray(x[:, :num_pilots]), rx_data=np.ascontiguousarray(x[:, num_pilots:]),
+                tx_train=np.ascontiguousarray(sym[:, :num_pilots]), tx_data=np.ascontiguousarray(sym[:, num_pilots:]),
+                bps=bps)
+
+
+def kernel_variance_for_sigma(sigma, num_antennas):
+    """Configuration value v such that the kernel is exp(-|u-d|^2 / (2 sigma^2)): the library
+    scales v by N^2 (cpp/lib/apsm/apsm_noma_detector.cu:79-80)."""
+    return sigma * sigma / float(num_antennas * num_antennas)
+
+
+# Workloads (sizes from the benchmark configurations).
+WORKLOADS = {
+    # 6 NOMA users, QPSK, 8 antennas, 10 dB; detect user 1; w=80, sigma=0.5, eps=0.01
+    "plain-cpu-8ant": dict(N=8, U_tx=6, users=[1], T=20000, M=80000, modulation="QPSK", snr_db=10.0, seed=1,
+                           large_scale_db=None, window=80, sigma=0.5, eB=0.01, dict_size=20000),
+    # 10 users, 16QAM, 16 antennas, 15 dB, gains U[-6,0] dB; all 10 users; w=160, Dc=4096
+    "plain-16ant-16qam": dict(N=16, U_tx=10, users=list(range(10)), T=200000, M=2000000, modulation="QAM16",
+                              snr_db=15.0, seed=2, large_scale_db=(-6.0, 0.0), window=160, sigma=0.5, eB=0.01,
+                              dict_size=4096),
+}
+
+
+def workload_config(w):
+    """Detector JSON configuration (reference keys) of a workload."""
+    return {"otype": "APSM", "windowSize": w["window"], "eB": w["eB"],
+            "gaussianKernelVariance": kernel_variance_for_sigma(w["sigma"], w["N"]),
+            "dictionarySize": w["dict_size"]}
