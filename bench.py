@@ -157,8 +157,7 @@ def main():
     est = d_est.cpu().numpy()
     est_c = est[..., 0] + 1j * est[..., 1]
     tx_data = data["tx_data"][w["users"]]
-    hd = datagen.hard_decision(est_c, data["bps"])
-    ser = float(np.mean(hd != tx_data))
+    ser = datagen.symbol_errors(est_c, tx_data, data["bps"]) / est_c.size
 
     tr_ms, dt_ms = float(np.mean(t_train_k)), float(np.mean(t_det_k))
     W2, H = 2 * w["window"], 2 * w["dict_size"]
@@ -208,10 +207,10 @@ def main():
         lls.synchronize()
         t_lls = time.perf_counter() - t0
         el = d_el.cpu().numpy()
-        hd_l = datagen.hard_decision(el[:, 0] + 1j * el[:, 1], data["bps"])
-        out["lls"] = {"user": w["users"][0], "ms": 1000 * t_lls,
-                      "ser": float(np.mean(hd_l != data["tx_data"][w["users"][0]])),
-                      "apsm_ser_same_user": float(np.mean(hd[0] != tx_data[0]))}
+        u0 = w["users"][0]
+        out["lls"] = {"user": u0, "ms": 1000 * t_lls,
+                      "ser": datagen.symbol_errors(el[:, 0] + 1j * el[:, 1], data["tx_data"][u0], data["bps"]) / M,
+                      "apsm_ser_same_user": datagen.symbol_errors(est_c[0], tx_data[0], data["bps"]) / M}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, data, args.cpu_seconds)
     print(json.dumps(out))

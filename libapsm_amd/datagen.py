@@ -59,6 +59,15 @@ def hard_decision(est, bps):
     return q(est.real) + 1j * q(est.imag)
 
 
+def symbol_errors(est, tx, bps):
+    """Number of symbols whose hard decision differs from the transmitted symbol (decision
+    regions of each real dimension; cpp/cli/common/util.cpp:62-110 with the real-part rule
+    applied to both dimensions)."""
+    s = mod_scale(bps)
+    d = np.abs(hard_decision(est, bps) - np.asarray(tx).astype(np.complex128))
+    return int(np.count_nonzero(d > 0.5 * s))
+
+
 def generate(num_antennas, num_users, num_pilots, num_data, modulation="QPSK", snr_db=10.0,
              seed=1, large_scale_db=None, normalize=True):
     """Returns dict(rx_train [N][T], rx_data [N][M], tx_train [U][T], tx_data [U][M]) complex64.
