@@ -6,6 +6,7 @@
 
 #include "apsm_capi.h"
 #include "apsm_device.h"
+#include "apsm_train2.h"
 
 #include <hip/hip_runtime.h>
 
@@ -233,7 +234,10 @@ struct apsm_handle
     int cur = 0;
     int64_t bufBaseC = 0; // absolute complex index of entry 0 of buf[cur]
     int64_t bufK = 0; // complex entries staged in buf[cur]
-    DevBuf<float> F, Gram, thOld, theta;
+    DevBuf<float> F, Gram, thOld, theta; // thOld: evicted-row linear share (per-call path) or thF (blocked path)
+    bool blocked = false; // blocked train path (register-resident window Gram)
+    int64_t a0Next = 0; // blocked path: frozen boundary after the last step (absolute real row)
+    DevBuf<float> Pbuf, Gact, EvF; // blocked path work buffers
     DevBuf<float> rx, tx, est; // staging for the host entry points
     DevBuf<int32_t> perm;
     lls_handle* lls = nullptr;
@@ -245,10 +249,12 @@ void resetDeviceState( apsm_handle* h )
 {
     const size_t nF = (size_t)h->U * h->w2, nG = (size_t)h->U * h->w2 * h->w2, nT = (size_t)h->U * h->L;
     HIP_OR_THROW( hipMemsetAsync( h->F.ptr, 0, nF * sizeof( float ), h->stream ) );
-    HIP_OR_THROW( hipMemsetAsync( h->Gram.ptr, 0, nG * sizeof( float ), h->stream ) );
+    if ( h->Gram.ptr )
+        HIP_OR_THROW( hipMemsetAsync( h->Gram.ptr, 0, nG * sizeof( float ), h->stream ) );
     HIP_OR_THROW( hipMemsetAsync( h->thOld.ptr, 0, nT * sizeof( float ), h->stream ) );
     HIP_OR_THROW( hipMemsetAsync( h->theta.ptr, 0, nT * sizeof( float ), h->stream ) );
     h->used = 0;
+    h->a0Next = 0;
     h->bufBaseC = 0;
     h->bufK = 0;
     h->anyTrain = false;
@@ -278,6 +284,208 @@ void validateParams( const apsm_params* p, uint32_t N, uint32_t U )
         break;
     default:
         throw ApsmError { APSM_ERR_ARG, "unknown detect version" };
+    }
+}
+
+// Blocked-path eligibility: single pass, no full-window start, window Gram fits the register
+// layouts (windowSize <= 160), at most kNewMax complex samples entering per step.
+bool blockedEligible( const apsm_params& p )
+{
+    return p.train_passes == 1 && !p.start_with_full_window && p.window_size <= 160 && ( p.sample_step == 2 || p.sample_step == 4 );
+}
+
+constexpr int64_t kBlockRowsMax = 256; // real rows entering per block
+
+template <int TP, int NEWPER>
+void launchSeq2( apsm_handle* h, const float* Z, const float* Zn, const float* Y, float* C, const apsm::Seq2Args& s2, size_t lds )
+{
+    if ( lds > 64 * 1024 )
+        HIP_OR_THROW( hipFuncSetAttribute( (const void*)apsm::apsm_train_seq2<TP, NEWPER>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds ) );
+    hipLaunchKernelGGL( ( apsm::apsm_train_seq2<TP, NEWPER> ), dim3( h->U ), dim3( apsm::kSeqThreads ), lds, h->stream, Z, Zn, Y, C,
+                        h->F.ptr, h->thOld.ptr, h->Pbuf.ptr, h->Gact.ptr, h->EvF.ptr, s2 );
+    HIP_OR_THROW( hipGetLastError() );
+}
+
+// Gram register triples per thread for a window of Wc complex slots (see apsm_train_seq2).
+int seq2Tp( int Wc )
+{
+    const int parts = apsm::kSeqThreads / Wc;
+    const int need = ( Wc + parts - 1 ) / parts;
+    for ( int tp : { 2, 4, 7, 10, 16, 27 } )
+        if ( tp >= need )
+            return tp;
+    return -1;
+}
+
+template <int UMAX>
+void launchEntry( apsm_handle* h, const float* Z, const float* Zn, const float* C, const float* thF, float* P, const apsm::EntryArgs& ea )
+{
+    hipLaunchKernelGGL( ( apsm::apsm_entry_frozen<UMAX> ), dim3( ea.nNewC ), dim3( apsm::kEntryThreads ), 0, h->stream, Z, Zn, C, thF, P,
+                        ea );
+    HIP_OR_THROW( hipGetLastError() );
+}
+
+// Runs the projection steps of one call in blocks (see apsm_train2.h).
+void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedOld, int64_t R2, int64_t rowBase, float wL, float wG,
+                   float g2 )
+{
+    const int64_t q = h->p.sample_step;
+    const int w2 = h->w2, Wc = w2 / 2, U = h->U;
+    const int newPer = (int)( q / 2 );
+    const int64_t nStepsTotal = R2 / q;
+    // block size: entering rows <= H - W + q (no active row leaves the dictionary inside a
+    // block), active samples <= kActMax, step package <= prefetch registers
+    int64_t steps = std::max<int64_t>( 1, std::min<int64_t>( kBlockRowsMax / q, ( h->H - w2 ) / q + 1 ) );
+    while ( steps > 1 && ( w2 + steps * q ) / 2 > apsm::kActMax )
+        steps--;
+    if ( ( w2 + steps * q ) / 2 > apsm::kActMax )
+        throw ApsmError { APSM_ERR_UNSUPPORTED, "window too large for the blocked train path" };
+    h->Pbuf.ensure( (size_t)U * steps * newPer * 2 );
+    h->Gact.ensure( (size_t)apsm::triIdx( ( w2 + steps * q ) / 2 ) * 3 );
+    h->EvF.ensure( (size_t)U * steps * w2 );
+
+    const int tp = seq2Tp( Wc );
+    if ( tp < 0 )
+        throw ApsmError { APSM_ERR_UNSUPPORTED, "window too large for the blocked train path" };
+    const int nParts = std::min( apsm::kSeqThreads / Wc, ( Wc + tp - 1 ) / tp );
+
+    for ( int64_t k0 = 0; k0 < nStepsTotal; k0 += steps )
+    {
+        const int64_t n = std::min( steps, nStepsTotal - k0 );
+        const int64_t sPrev = usedOld + k0 * q, sFirst = sPrev + q, sLast = sPrev + n * q;
+        const int64_t a0 = sFirst - std::min<int64_t>( w2, sFirst );
+        const int64_t sNext = sLast + q;
+        const int64_t a0Next = sNext - std::min<int64_t>( w2, sNext );
+        const int nNewC = (int)( n * newPer );
+        const int nActC = (int)( ( sLast - a0 ) / 2 );
+
+        apsm::EntryArgs ea {};
+        ea.N = h->N;
+        ea.L = h->L;
+        ea.U = U;
+        ea.q = (int)q;
+        ea.rowBase = rowBase;
+        ea.newC0 = ( sPrev - rowBase ) / 2;
+        ea.sFirst = sFirst;
+        ea.H = h->H;
+        ea.a0 = a0;
+        ea.ldYC = ld;
+        ea.nNewC = nNewC;
+        ea.wL = wL;
+        ea.wG = wG;
+        ea.g2 = g2;
+        for ( int u0 = 0; u0 < U; u0 += 16 )
+        {
+            // users in groups of 16 share one launch
+            apsm::EntryArgs eu = ea;
+            eu.U = std::min( 16, U - u0 );
+            const float* Cg = b.C.ptr + (size_t)u0 * ld;
+            const float* thg = h->thOld.ptr + (size_t)u0 * h->L;
+            float* Pg = h->Pbuf.ptr + (size_t)u0 * nNewC * 2;
+            if ( eu.U <= 1 )
+                launchEntry<1>( h, b.Z.ptr, b.Zn.ptr, Cg, thg, Pg, eu );
+            else if ( eu.U <= 2 )
+                launchEntry<2>( h, b.Z.ptr, b.Zn.ptr, Cg, thg, Pg, eu );
+            else if ( eu.U <= 4 )
+                launchEntry<4>( h, b.Z.ptr, b.Zn.ptr, Cg, thg, Pg, eu );
+            else if ( eu.U <= 8 )
+                launchEntry<8>( h, b.Z.ptr, b.Zn.ptr, Cg, thg, Pg, eu );
+            else
+                launchEntry<16>( h, b.Z.ptr, b.Zn.ptr, Cg, thg, Pg, eu );
+        }
+
+        apsm::GramArgs ga {};
+        ga.N = h->N;
+        ga.L = h->L;
+        ga.c0 = ( a0 - rowBase ) / 2;
+        ga.nA = nActC;
+        ga.wL = wL;
+        ga.wG = wG;
+        ga.g2 = g2;
+        const int64_t pairs = apsm::triIdx( nActC );
+        hipLaunchKernelGGL( apsm::apsm_active_gram, dim3( gridFor( pairs, 256 ) ), dim3( 256 ), 0, h->stream, b.Z.ptr, b.Zn.ptr,
+                            h->Gact.ptr, ga );
+        HIP_OR_THROW( hipGetLastError() );
+
+        const int64_t loLast = sLast > h->H ? sLast - h->H : 0, loPrev = sPrev > h->H ? sPrev - h->H : 0;
+        const int hasEv = loLast > loPrev ? 1 : 0;
+        if ( hasEv )
+        {
+            apsm::EvArgs ev {};
+            ev.N = h->N;
+            ev.L = h->L;
+            ev.U = U;
+            ev.q = (int)q;
+            ev.w2 = w2;
+            ev.nSteps = (int)n;
+            ev.rowBase = rowBase;
+            ev.sPrev = sPrev;
+            ev.sFirst = sFirst;
+            ev.H = h->H;
+            ev.ldYC = ld;
+            ev.wG = wG;
+            ev.g2 = g2;
+            hipLaunchKernelGGL( apsm::apsm_evict_corr, dim3( gridFor( n * Wc, 256 ) ), dim3( 256 ), 0, h->stream, b.Z.ptr, b.Zn.ptr,
+                                b.C.ptr, h->EvF.ptr, ev );
+            HIP_OR_THROW( hipGetLastError() );
+        }
+
+        apsm::Seq2Args s2 {};
+        s2.N = h->N;
+        s2.L = h->L;
+        s2.U = U;
+        s2.w2 = w2;
+        s2.Wc = Wc;
+        s2.q = (int)q;
+        s2.nSteps = (int)n;
+        s2.hasEv = hasEv;
+        s2.nParts = nParts;
+        s2.ramp = a0 == 0 ? 1 : 0;
+        s2.a0modW = (int)( a0 % w2 );
+        s2.sFirstRel = (int)( sFirst - a0 );
+        s2.nNewC = nNewC;
+        s2.nActC = nActC;
+        s2.newRel0 = (int)( ( sPrev - a0 ) / 2 );
+        s2.nFr = (int)( ( a0Next - a0 ) / 2 );
+        s2.pkFloats = newPer * 3 * nActC + 2 * newPer + w2;
+        s2.ldYC = ld;
+        s2.wL = wL;
+        s2.wG = wG;
+        s2.eB = h->p.eB;
+        const size_t lds = sizeof( float )
+            * ( (size_t)w2 + 2 * apsm::kSeqThreads + 4 * apsm::kActMax + apsm::kActMax + apsm::kSeqWaves * 2 * apsm::kNewMax
+                + 2 * apsm::kSeqThreads + 2 * (size_t)s2.pkFloats + (size_t)newPer * 3 * apsm::kSeqThreads );
+        const int64_t aR = a0 - rowBase; // local real row of the first active row
+        const float* Za = b.Z.ptr + ( aR / 2 ) * h->L;
+        const float* Zna = b.Zn.ptr + aR / 2;
+        const float* Ya = b.Y.ptr + aR;
+        float* Ca = b.C.ptr + aR;
+        if ( newPer == 1 )
+        {
+            switch ( tp )
+            {
+            case 2: launchSeq2<2, 1>( h, Za, Zna, Ya, Ca, s2, lds ); break;
+            case 4: launchSeq2<4, 1>( h, Za, Zna, Ya, Ca, s2, lds ); break;
+            case 7: launchSeq2<7, 1>( h, Za, Zna, Ya, Ca, s2, lds ); break;
+            case 10: launchSeq2<10, 1>( h, Za, Zna, Ya, Ca, s2, lds ); break;
+            case 16: launchSeq2<16, 1>( h, Za, Zna, Ya, Ca, s2, lds ); break;
+            default: launchSeq2<27, 1>( h, Za, Zna, Ya, Ca, s2, lds ); break;
+            }
+        }
+        else
+        {
+            switch ( tp )
+            {
+            case 2: launchSeq2<2, 2>( h, Za, Zna, Ya, Ca, s2, lds ); break;
+            case 4: launchSeq2<4, 2>( h, Za, Zna, Ya, Ca, s2, lds ); break;
+            case 7: launchSeq2<7, 2>( h, Za, Zna, Ya, Ca, s2, lds ); break;
+            case 10: launchSeq2<10, 2>( h, Za, Zna, Ya, Ca, s2, lds ); break;
+            case 16: launchSeq2<16, 2>( h, Za, Zna, Ya, Ca, s2, lds ); break;
+            default: launchSeq2<27, 2>( h, Za, Zna, Ya, Ca, s2, lds ); break;
+            }
+        }
+        h->a0Next = a0Next;
     }
 }
 
@@ -390,9 +598,14 @@ void trainDev( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, 
         * ( 2 * (size_t)h->w2 + L + apsm::kNewMax * L + apsm::kNewMax + apsm::kSeqWaves * 2 * apsm::kNewMax
             + apsm::kSeqWaves * apsm::kWave );
     HIP_OR_THROW( hipEventRecord( h->ev0, h->stream ) );
-    hipLaunchKernelGGL( apsm::apsm_train_seq, dim3( U ), dim3( apsm::kSeqThreads ), lds, h->stream, nb.Z.ptr, nb.Zn.ptr, nb.Y.ptr,
-                        nb.C.ptr, h->F.ptr, h->Gram.ptr, h->thOld.ptr, a );
-    HIP_OR_THROW( hipGetLastError() );
+    if ( h->blocked )
+        trainBlocked( h, nb, ld, (int64_t)h->used, 2 * T, a.rowBase, wL, wG, a.g2 );
+    else
+    {
+        hipLaunchKernelGGL( apsm::apsm_train_seq, dim3( U ), dim3( apsm::kSeqThreads ), lds, h->stream, nb.Z.ptr, nb.Zn.ptr, nb.Y.ptr,
+                            nb.C.ptr, h->F.ptr, h->Gram.ptr, h->thOld.ptr, a );
+        HIP_OR_THROW( hipGetLastError() );
+    }
     HIP_OR_THROW( hipEventRecord( h->ev1, h->stream ) );
     h->trainTimed = true;
 
@@ -406,7 +619,7 @@ void trainDev( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, 
     h->bufK = K;
 
     const int64_t newUsedC = (int64_t)( h->used / 2 );
-    const int64_t lo = std::max<int64_t>( 0, newUsedC - h->Hc );
+    const int64_t lo = h->blocked ? std::min<int64_t>( h->a0Next / 2, newUsedC ) : std::max<int64_t>( 0, newUsedC - h->Hc );
     hipLaunchKernelGGL( apsm::apsm_theta_full, dim3( U ), dim3( 256 ), 0, h->stream, nb.Z.ptr, nb.C.ptr, h->thOld.ptr,
                         h->theta.ptr, N, lo - h->bufBaseC, newUsedC - h->bufBaseC, ld );
     HIP_OR_THROW( hipGetLastError() );
@@ -532,13 +745,15 @@ int apsm_create( const apsm_params* p, uint32_t num_antennas, uint32_t num_users
             h->w2 = 2 * (int)p->window_size;
             h->Hc = p->dictionary_size;
             h->H = 2 * h->Hc;
+            h->blocked = blockedEligible( *p );
             HIP_OR_THROW( hipStreamCreateWithFlags( &h->stream, hipStreamNonBlocking ) );
             HIP_OR_THROW( hipEventCreate( &h->ev0 ) );
             HIP_OR_THROW( hipEventCreate( &h->ev1 ) );
             HIP_OR_THROW( hipEventCreate( &h->ev2 ) );
             HIP_OR_THROW( hipEventCreate( &h->ev3 ) );
             h->F.ensure( (size_t)h->U * h->w2 );
-            h->Gram.ensure( (size_t)h->U * h->w2 * h->w2 );
+            if ( !h->blocked )
+                h->Gram.ensure( (size_t)h->U * h->w2 * h->w2 );
             h->thOld.ensure( (size_t)h->U * h->L );
             h->theta.ensure( (size_t)h->U * h->L );
             resetDeviceState( h );
@@ -563,6 +778,9 @@ void apsm_destroy( apsm_handle* h )
         b.release();
     h->F.release();
     h->Gram.release();
+    h->Pbuf.release();
+    h->Gact.release();
+    h->EvF.release();
     h->thOld.release();
     h->theta.release();
     h->rx.release();

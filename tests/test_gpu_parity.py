@@ -72,6 +72,12 @@ CASES = [
     ("n64", dict(windowSize=20, dictionarySize=300, eB=0.01, gaussianKernelVariance=0.25 / 4096), 64, 300),
     ("n5-odd", dict(windowSize=7, dictionarySize=100, eB=0.01, gaussianKernelVariance=0.1), 5, 120),
     ("w512", dict(windowSize=512, dictionarySize=600, eB=0.01, gaussianKernelVariance=0.25 / 64), 8, 700),
+    ("q4-fifo", dict(windowSize=12, dictionarySize=50, eB=0.01, gaussianKernelVariance=0.1, sampleStep=4), 4, 300),
+    ("w64", dict(windowSize=64, dictionarySize=200, eB=0.01, gaussianKernelVariance=0.25 / 64), 8, 500),
+    ("w100-fifo", dict(windowSize=100, dictionarySize=150, eB=0.01, gaussianKernelVariance=0.25 / 64), 8, 600),
+    ("w128", dict(windowSize=128, dictionarySize=700, eB=0.01, gaussianKernelVariance=0.25 / 256), 16, 600),
+    ("w160-evict", dict(windowSize=160, dictionarySize=400, eB=0.01, gaussianKernelVariance=0.25 / 256), 16, 1500),
+    ("w40-q4", dict(windowSize=40, dictionarySize=90, eB=0.001, gaussianKernelVariance=0.25 / 64, sampleStep=4), 8, 500),
 ]
 
 
