@@ -296,15 +296,33 @@ bool blockedEligible( const apsm_params& p )
 
 constexpr int64_t kBlockRowsMax = 256; // real rows entering per block
 
-template <int TP, int NEWPER>
+template <int TP, int NEWPER, int PMAX>
 void launchSeq2( apsm_handle* h, const float* Z, const float* Zn, const float* Y, float* C, const apsm::Seq2Args& s2, size_t lds )
 {
+    if ( s2.nParts > PMAX || PMAX * s2.Wc > apsm::kRedF )
+        throw ApsmError { APSM_ERR_UNSUPPORTED, "window layout does not fit the blocked train path" };
     if ( lds > 64 * 1024 )
-        HIP_OR_THROW( hipFuncSetAttribute( (const void*)apsm::apsm_train_seq2<TP, NEWPER>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds ) );
-    hipLaunchKernelGGL( ( apsm::apsm_train_seq2<TP, NEWPER> ), dim3( h->U ), dim3( apsm::kSeqThreads ), lds, h->stream, Z, Zn, Y, C,
-                        h->F.ptr, h->thOld.ptr, h->Pbuf.ptr, h->Gact.ptr, h->EvF.ptr, s2 );
+        HIP_OR_THROW( hipFuncSetAttribute( (const void*)apsm::apsm_train_seq2<TP, NEWPER, PMAX>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds ) );
+    hipLaunchKernelGGL( ( apsm::apsm_train_seq2<TP, NEWPER, PMAX> ), dim3( h->U ), dim3( apsm::kSeqThreads ), lds, h->stream, Z, Zn,
+                        Y, C, h->F.ptr, h->thOld.ptr, h->Pbuf.ptr, h->Gact.ptr, h->EvF.ptr, s2 );
     HIP_OR_THROW( hipGetLastError() );
+}
+
+// Instantiations: (register triples per thread, upper bound of the t-slot part count).
+template <int NEWPER>
+void launchSeq2Tp( apsm_handle* h, int tp, const float* Z, const float* Zn, const float* Y, float* C, const apsm::Seq2Args& s2,
+                   size_t lds )
+{
+    switch ( tp )
+    {
+    case 2: launchSeq2<2, NEWPER, 24>( h, Z, Zn, Y, C, s2, lds ); break;
+    case 4: launchSeq2<4, NEWPER, 16>( h, Z, Zn, Y, C, s2, lds ); break;
+    case 7: launchSeq2<7, NEWPER, 12>( h, Z, Zn, Y, C, s2, lds ); break;
+    case 10: launchSeq2<10, NEWPER, 10>( h, Z, Zn, Y, C, s2, lds ); break;
+    case 16: launchSeq2<16, NEWPER, 8>( h, Z, Zn, Y, C, s2, lds ); break;
+    default: launchSeq2<27, NEWPER, 6>( h, Z, Zn, Y, C, s2, lds ); break;
+    }
 }
 
 // Gram register triples per thread for a window of Wc complex slots (see apsm_train_seq2).
@@ -349,6 +367,8 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
     if ( tp < 0 )
         throw ApsmError { APSM_ERR_UNSUPPORTED, "window too large for the blocked train path" };
     const int nParts = std::min( apsm::kSeqThreads / Wc, ( Wc + tp - 1 ) / tp );
+    if ( nParts > apsm::kMaxParts || nParts * tp < Wc )
+        throw ApsmError { APSM_ERR_UNSUPPORTED, "window layout does not fit the blocked train path" };
 
     for ( int64_t k0 = 0; k0 < nStepsTotal; k0 += steps )
     {
@@ -453,38 +473,16 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
         s2.wL = wL;
         s2.wG = wG;
         s2.eB = h->p.eB;
-        const size_t lds = sizeof( float )
-            * ( (size_t)w2 + 2 * apsm::kSeqThreads + 4 * apsm::kActMax + apsm::kActMax + apsm::kSeqWaves * 2 * apsm::kNewMax
-                + 2 * apsm::kSeqThreads + 2 * (size_t)s2.pkFloats + (size_t)newPer * 3 * apsm::kSeqThreads );
+        const size_t lds = apsm::seq2LdsBytes( newPer, w2, tp );
         const int64_t aR = a0 - rowBase; // local real row of the first active row
         const float* Za = b.Z.ptr + ( aR / 2 ) * h->L;
         const float* Zna = b.Zn.ptr + aR / 2;
         const float* Ya = b.Y.ptr + aR;
         float* Ca = b.C.ptr + aR;
         if ( newPer == 1 )
-        {
-            switch ( tp )
-            {
-            case 2: launchSeq2<2, 1>( h, Za, Zna, Ya, Ca, s2, lds ); break;
-            case 4: launchSeq2<4, 1>( h, Za, Zna, Ya, Ca, s2, lds ); break;
-            case 7: launchSeq2<7, 1>( h, Za, Zna, Ya, Ca, s2, lds ); break;
-            case 10: launchSeq2<10, 1>( h, Za, Zna, Ya, Ca, s2, lds ); break;
-            case 16: launchSeq2<16, 1>( h, Za, Zna, Ya, Ca, s2, lds ); break;
-            default: launchSeq2<27, 1>( h, Za, Zna, Ya, Ca, s2, lds ); break;
-            }
-        }
+            launchSeq2Tp<1>( h, tp, Za, Zna, Ya, Ca, s2, lds );
         else
-        {
-            switch ( tp )
-            {
-            case 2: launchSeq2<2, 2>( h, Za, Zna, Ya, Ca, s2, lds ); break;
-            case 4: launchSeq2<4, 2>( h, Za, Zna, Ya, Ca, s2, lds ); break;
-            case 7: launchSeq2<7, 2>( h, Za, Zna, Ya, Ca, s2, lds ); break;
-            case 10: launchSeq2<10, 2>( h, Za, Zna, Ya, Ca, s2, lds ); break;
-            case 16: launchSeq2<16, 2>( h, Za, Zna, Ya, Ca, s2, lds ); break;
-            default: launchSeq2<27, 2>( h, Za, Zna, Ya, Ca, s2, lds ); break;
-            }
-        }
+            launchSeq2Tp<2>( h, tp, Za, Zna, Ya, Ca, s2, lds );
         h->a0Next = a0Next;
     }
 }

@@ -25,6 +25,13 @@ namespace apsm {
 
 constexpr int kActMax = 512; // active complex samples per block (host guarantees)
 constexpr int kEntryThreads = 256;
+constexpr int kMaxParts = 32; // t-slot parts of the register Gram (1024 / Wc for Wc >= 32)
+constexpr int kRedF = 2048; // per-part f partial slots (>= PMAX * Wc of every instantiation)
+// Thread roles in the sequencer's entering phase: [0, w2) reduce f (w2 <= 320), [kColBase,
+// kColBase + nParts*TP) stage the Gram column (<= 192 slots), [kDotBase, kDotBase + c0) take
+// the active rows' share of the entering f (c0 < kActMax).
+constexpr int kColBase = 320;
+constexpr int kDotBase = 512;
 
 __host__ __device__ __forceinline__ int64_t triIdx( int64_t i ) { return i * ( i + 1 ) / 2; }
 
@@ -171,7 +178,8 @@ struct EvArgs
 
 // EvF[u][k][2*pc + part]: Gaussian share of the rows leaving the dictionary at step k, for
 // the window row at position 2*pc + part (row s_k - W_k + 2*pc + part) if it entered before
-// step k; zero otherwise. One thread per (step, complex window position).
+// step k; zero otherwise. One thread per (step, complex window position); the kernel values
+// of the (at most kNewMax) leaving samples are computed once and contracted for every user.
 __global__ void apsm_evict_corr( const float* __restrict__ Z, const float* __restrict__ Zn, const float* __restrict__ C,
                                  float* __restrict__ EvF, const EvArgs a )
 {
@@ -186,23 +194,38 @@ __global__ void apsm_evict_corr( const float* __restrict__ Z, const float* __res
     const int64_t lo = s > a.H ? s - a.H : 0, loP = sp > a.H ? sp - a.H : 0;
     const int64_t r = s - W + 2 * pc; // absolute real row (a-row of the complex sample)
     const bool live = ( 2 * pc < W ) && ( r < sp ) && ( lo > loP );
+    const int64_t e0 = ( loP - a.rowBase ) / 2;
+    const int ne = live ? (int)( ( lo - loP ) / 2 ) : 0; // <= kNewMax
+    float K0[ kNewMax ], K1[ kNewMax ], K2[ kNewMax ];
+    if ( live )
+    {
+        const int64_t vv = ( r - a.rowBase ) / 2;
+#pragma unroll
+        for ( int i = 0; i < kNewMax; i++ )
+        {
+            K0[ i ] = K1[ i ] = K2[ i ] = 0.f;
+            if ( i < ne )
+            {
+                float p, sp2;
+                pairDots( Z + vv * a.L, Z + ( e0 + i ) * a.L, a.N, p, sp2 );
+                const float nn = Zn[ vv ] + Zn[ e0 + i ];
+                K0[ i ] = gaussFromDist( nn - 2.f * p, a.g2 );
+                K1[ i ] = gaussFromDist( nn - 2.f * sp2, a.g2 );
+                K2[ i ] = gaussFromDist( nn + 2.f * sp2, a.g2 );
+            }
+        }
+    }
     for ( int u = 0; u < a.U; u++ )
     {
         float dFa = 0.f, dFb = 0.f;
-        if ( live )
+#pragma unroll
+        for ( int i = 0; i < kNewMax; i++ )
         {
-            const int64_t v = ( r - a.rowBase ) / 2;
-            for ( int64_t e = ( loP - a.rowBase ) / 2; e < ( lo - a.rowBase ) / 2; e++ )
+            if ( i < ne )
             {
-                float p, sp2;
-                pairDots( Z + v * a.L, Z + e * a.L, a.N, p, sp2 );
-                const float nn = Zn[ v ] + Zn[ e ];
-                const float E0 = gaussFromDist( nn - 2.f * p, a.g2 );
-                const float E1 = gaussFromDist( nn - 2.f * sp2, a.g2 );
-                const float E2 = gaussFromDist( nn + 2.f * sp2, a.g2 );
-                const float ca = C[ u * a.ldYC + 2 * e ], cb = C[ u * a.ldYC + 2 * e + 1 ];
-                dFa += ca * E0 + cb * E1;
-                dFb += ca * E2 + cb * E0;
+                const float ca = C[ u * a.ldYC + 2 * ( e0 + i ) ], cb = C[ u * a.ldYC + 2 * ( e0 + i ) + 1 ];
+                dFa += ca * K0[ i ] + cb * K1[ i ];
+                dFb += ca * K2[ i ] + cb * K0[ i ];
             }
         }
         float* o = EvF + ( (int64_t)u * a.nSteps + kk ) * a.w2;
@@ -226,6 +249,38 @@ struct Seq2Args
     float wL, wG, eB;
 };
 
+// LDS-only workgroup barrier: waits for this thread's LDS operations, not for outstanding
+// global loads (the step packages stream in asynchronously across barriers).
+__device__ __forceinline__ void ldsBarrier() { asm volatile( "s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory" ); }
+
+// Asynchronous global -> LDS copy of one dword per lane (lane i writes ldsWaveBase + 4*i).
+// M0 carries the wave's LDS destination (one wait state between the M0 write and the load);
+// the sequencer kernel uses M0 for nothing else.
+__device__ __forceinline__ void dmaDword( const float* src, uint32_t ldsWaveBase )
+{
+    asm volatile( "s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off" ::"s"( ldsWaveBase ), "v"( src ) : "memory" );
+}
+
+template <int N>
+__device__ __forceinline__ void waitVm()
+{
+    asm volatile( "s_waitcnt vmcnt(%0)" ::"n"( N ) : "memory" );
+}
+
+// s_waitcnt vmcnt(n) for a wave-uniform n <= 5
+__device__ __forceinline__ void waitVmN( int n )
+{
+    switch ( n )
+    {
+    case 0: waitVm<0>(); break;
+    case 1: waitVm<1>(); break;
+    case 2: waitVm<2>(); break;
+    case 3: waitVm<3>(); break;
+    case 4: waitVm<4>(); break;
+    default: waitVm<5>(); break;
+    }
+}
+
 // Register-resident window Gram. Thread tid owns u-slot v = tid % Wc and t-slots
 // [part*TP, part*TP + TP) with part = tid / Wc; per (t, v) it keeps
 // A = kappa(a_t,a_v), B = kappa(a_t,b_v), C = kappa(b_t,a_v).
@@ -233,18 +288,21 @@ struct Seq2Args
 // 32 bits; slots follow the absolute row (a0modW + rel) % w2, as the f array does.
 // Pointers are offset by the host: Z, Zn to the first active complex sample; Y, C to the
 // first active real row of user 0.
-// Step package in LDS (double-buffered, filled one step ahead):
-//   [0, NEWPER*3*nAct)       Gram rows of the entering samples
+// Step package (triple-buffered in LDS, copied two steps ahead by global->LDS DMA):
+//   [0, NEWPER*3*nAct)       Gram rows of the entering samples (row i at i*3*nAct)
 //   [+0, +2*NEWPER)          their frozen shares P
 //   [+0, +w2)                eviction shares of the window rows
-template <int TP, int NEWPER>
+template <int TP, int NEWPER, int PMAX>
 __global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* __restrict__ Z, const float* __restrict__ Zn,
                                                                 const float* __restrict__ Y, float* __restrict__ C,
                                                                 float* __restrict__ Fg, float* __restrict__ thF,
                                                                 const float* __restrict__ P, const float* __restrict__ Gact,
                                                                 const float* __restrict__ EvF, const Seq2Args a )
 {
-    constexpr int kRowRegs = ( NEWPER * 3 * kActMax + kSeqThreads - 1 ) / kSeqThreads;
+    constexpr int kDpw = ( NEWPER * 3 * kActMax + 2 * NEWPER + kSeqThreads + kSeqThreads - 1 ) / kSeqThreads; // DMAs per wave per step
+    constexpr int kPkCap = kDpw * kSeqThreads; // floats per package buffer
+    constexpr int TPA = ( TP + 3 ) & ~3; // column segment per part, padded for float4 reads
+    constexpr int kColStride = kMaxParts * TPA; // floats per column array (A, B or C)
     const int u = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & ( kWave - 1 ), wave = tid / kWave;
     const int Wc = a.Wc, w2 = a.w2, L = a.L;
@@ -255,34 +313,71 @@ __global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* _
     const int cMod = a.a0modW >> 1; // complex slot of relative complex index 0
     const int rowFloats = 3 * nAct;
     const int pkRows = NEWPER * rowFloats;
+    const int pkF = pkRows + 2 * NEWPER + w2;
 
     extern __shared__ float sm[];
-    float* sF = sm; // [w2]
+    float* sPk = sm; // [3][kPkCap] (first, so the DMA destinations are 256-byte aligned)
+    float* sF = sPk + 3 * kPkCap; // [w2]
     float* sB = sF + w2; // [2 * kSeqThreads]
     float* sC = sB + 2 * kSeqThreads; // [2 * kActMax]
     float* sY = sC + 2 * kActMax; // [2 * kActMax]
-    float* sZn = sY + 2 * kActMax; // [kActMax]
-    float* sRedE = sZn + kActMax; // [kSeqWaves][2 * kNewMax]
-    float* sRedF = sRedE + kSeqWaves * 2 * kNewMax; // [2 * kSeqThreads]
-    float* sPk = sRedF + 2 * kSeqThreads; // [2][pkFloats]
-    float* sCol = sPk + 2 * a.pkFloats; // [NEWPER][3][kSeqThreads]: Gram column of each entering sample by t-slot
+    float* sInv = sY + 2 * kActMax; // [kActMax]: 1 / (wL |x|^2 + wG)
+    float* sRedE = sInv + kActMax; // [kSeqWaves][2 * kNewMax]
+    float* sRedF = sRedE + kSeqWaves * 2 * kNewMax; // [2 * kRedF]: per-part f partials (zeros beyond nParts)
+    float* sCol = sRedF + 2 * kRedF; // [NEWPER][3][kColStride]
 
     const float* Yu = Y + u * a.ldYC;
     float* Cu = C + u * a.ldYC;
     const float* Pu = P + (int64_t)u * a.nNewC * 2;
     const float* EvU = EvF + (int64_t)u * a.nSteps * w2;
+    const float* gEnd = Gact + ( nAct * ( nAct + 1 ) / 2 ) * 3 - 1;
+
+    // issue the package of step kk into buffer kk % 3; wave w copies 64-float chunks
+    // w, w + 16, ... of the package (myChunks of them)
+    const int nChunks = ( pkF + kWave - 1 ) / kWave;
+    const int myChunks = wave < nChunks ? ( nChunks - wave + kSeqWaves - 1 ) / kSeqWaves : 0;
+    auto pkIssue = [ & ]( int kk ) {
+        const int c0 = a.newRel0 + kk * NEWPER;
+        float* buf = sPk + ( kk % 3 ) * kPkCap;
+#pragma unroll 1
+        for ( int d = 0; d < myChunks; d++ )
+        {
+            const int chunk = wave + d * kSeqWaves;
+            const int e = chunk * kWave + lane;
+            const float* src = Gact;
+            if ( e < pkRows )
+            {
+                const int i = NEWPER == 1 ? 0 : e / rowFloats;
+                const int t = c0 + i;
+                src = Gact + ( t * ( t + 1 ) / 2 ) * 3 + ( e - i * rowFloats );
+                src = src > gEnd ? gEnd : src;
+            }
+            else if ( e < pkRows + 2 * NEWPER )
+                src = Pu + kk * NEWPER * 2 + ( e - pkRows );
+            else if ( e < pkF )
+                src = EvU + kk * w2 + ( e - pkRows - 2 * NEWPER );
+            const uint32_t dst = (uint32_t)(uintptr_t)( __attribute__( ( address_space( 3 ) ) ) float* )( buf + chunk * kWave );
+            dmaDword( src, __builtin_amdgcn_readfirstlane( dst ) );
+        }
+    };
+
+    pkIssue( 0 );
+    if ( a.nSteps > 1 )
+        pkIssue( 1 );
 
     for ( int i = tid; i < w2; i += kSeqThreads )
         sF[ i ] = Fg[ (int64_t)u * w2 + i ];
     for ( int i = tid; i < 2 * kSeqThreads; i += kSeqThreads )
         sB[ i ] = 0.f;
+    for ( int i = tid; i < 2 * kRedF; i += kSeqThreads )
+        sRedF[ i ] = 0.f;
     for ( int i = tid; i < 2 * nAct; i += kSeqThreads )
     {
         sC[ i ] = Cu[ i ];
         sY[ i ] = Yu[ i ];
     }
     for ( int i = tid; i < nAct; i += kSeqThreads )
-        sZn[ i ] = Zn[ i ];
+        sInv[ i ] = 1.f / ( a.wL * Zn[ i ] + a.wG );
 
     // window Gram of the rows already in the window (active rows entered before the block);
     // the row in complex slot x, newest first from kLast, is kLast - ((pLast - x) mod Wc)
@@ -313,85 +408,59 @@ __global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* _
                 __builtin_amdgcn_sched_barrier( 0 );
         }
     }
-
-    float pfRow[ kRowRegs ];
-    float pfP = 0.f, pfE = 0.f;
-    auto pkLoad = [ & ]( int kk ) {
-        const int c0 = a.newRel0 + kk * NEWPER;
-        const float* rowBase = Gact + ( c0 * ( c0 + 1 ) / 2 ) * 3; // row of sample c0
-#pragma unroll
-        for ( int r = 0; r < kRowRegs; r++ )
-        {
-            const int e = tid + r * kSeqThreads;
-            float val = 0.f;
-            if ( e < pkRows )
-            {
-                // rows of consecutive samples c0, c0+1 are contiguous in the triangle, but the
-                // package pads each to rowFloats
-                const int i = NEWPER == 1 ? 0 : e / rowFloats;
-                const int off = e - i * rowFloats;
-                const int t = c0 + i;
-                if ( off < 3 * ( t + 1 ) )
-                    val = rowBase[ i * ( 3 * c0 + 3 ) + off ];
-            }
-            pfRow[ r ] = val;
-        }
-        pfP = tid < 2 * NEWPER ? Pu[ kk * NEWPER * 2 + tid ] : 0.f;
-        pfE = ( a.hasEv && tid < w2 ) ? EvU[ kk * w2 + tid ] : 0.f;
-    };
-    auto pkStore = [ & ]( float* dst ) {
-#pragma unroll
-        for ( int r = 0; r < kRowRegs; r++ )
-        {
-            const int e = tid + r * kSeqThreads;
-            if ( e < pkRows )
-                dst[ e ] = pfRow[ r ];
-        }
-        if ( tid < 2 * NEWPER )
-            dst[ pkRows + tid ] = pfP;
-        if ( tid < w2 )
-            dst[ pkRows + 2 * NEWPER + tid ] = pfE;
-    };
-    pkLoad( 0 );
-    pkStore( sPk );
+    waitVm<0>();
     __syncthreads();
 
-    const int pkF = pkRows + 2 * NEWPER + w2;
     for ( int kk = 0; kk < a.nSteps; kk++ )
     {
-        const float* pk = sPk + ( kk & 1 ) * pkF;
-        if ( kk + 1 < a.nSteps )
-            pkLoad( kk + 1 );
+        const float* pk = sPk + ( kk % 3 ) * kPkCap;
         const int sRel = a.sFirstRel + kk * a.q; // s - a0
         const int spRel = sRel - a.q;
         const int W = a.ramp ? ( sRel < w2 ? sRel : w2 ) : w2;
         const int c0 = a.newRel0 + kk * NEWPER; // first entering complex sample
-        int sBase = ( a.a0modW + sRel ) % w2; // slot of row s - w2 (== slot of row s)
+        const int sBase = ( a.a0modW + sRel ) % w2; // slot of row s (== slot of row s - w2)
+        // the package two steps ahead goes into the buffer read one step ago
+        if ( kk + 2 < a.nSteps )
+            pkIssue( kk + 2 );
 
-        // 1. rows leaving the dictionary: subtract their share from rows already in the window
-        if ( a.hasEv && tid < W )
+        // 1. previous step's Gram product lands in f; rows leaving the dictionary subtract their
+        //    share from the rows that were already in the window. Thread tid owns slot tid.
+        if ( tid < w2 )
         {
-            const int r = sRel - W + tid;
-            if ( r < spRel )
+            const int cv = tid >> 1, pt = tid & 1;
+            // parts beyond nParts hold zeros: the sum runs over PMAX without guards so the
+            // loads issue back to back
+            float acc = 0.f;
+#pragma unroll
+            for ( int t = 0; t < PMAX; t++ )
+                acc += sRedF[ 2 * ( t * Wc + cv ) + pt ];
+            if ( a.hasEv )
             {
-                int sl = sBase - W + tid;
-                sl += sl < 0 ? w2 : 0;
-                sF[ sl ] -= pk[ pkRows + 2 * NEWPER + tid ];
+                int pos = tid - sBase; // position of this slot's row among the last w2 rows
+                pos += pos < 0 ? w2 : 0;
+                const int r = sRel - w2 + pos;
+                const int wpos = r - ( sRel - W );
+                if ( wpos >= 0 && r < spRel )
+                    acc -= pk[ pkRows + 2 * NEWPER + wpos ];
             }
+            sF[ tid ] += acc;
         }
-        // 2. entering samples: share of the active rows with non-zero c, and Gram registers
+        // 2. entering samples: share of the active rows with non-zero c, and Gram entries
         {
+            // threads [512, 512 + c0) (waves 8..15) take one active sample each, so this runs
+            // beside the f reduction (waves 0..4) and the column staging (waves 5..7)
             float ea[ NEWPER ], eb[ NEWPER ];
 #pragma unroll
             for ( int i = 0; i < NEWPER; i++ )
                 ea[ i ] = eb[ i ] = 0.f;
-            if ( tid < c0 )
+            const int jd = tid - kDotBase;
+            if ( jd >= 0 && jd < c0 )
             {
-                const float ca = sC[ 2 * tid ], cb = sC[ 2 * tid + 1 ];
+                const float ca = sC[ 2 * jd ], cb = sC[ 2 * jd + 1 ];
 #pragma unroll
                 for ( int i = 0; i < NEWPER; i++ )
                 {
-                    const float* g = pk + i * rowFloats + 3 * tid;
+                    const float* g = pk + i * rowFloats + 3 * jd;
                     ea[ i ] = ca * g[ 0 ] + cb * g[ 1 ];
                     eb[ i ] = ca * g[ 2 ] + cb * g[ 0 ];
                 }
@@ -419,20 +488,24 @@ __global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* _
             {
                 const int k = c0 + i;
                 const int p = ( cMod + k ) % Wc;
-                float* col = sCol + i * 3 * kSeqThreads;
-                // column: sCol[i][.][tau] = (kappa(a_tau,a_k), kappa(a_tau,b_k), kappa(b_tau,a_k))
-                if ( tid < a.nParts * TP )
+                float* col = sCol + i * 3 * kColStride;
+                // column: sCol[i][.][part*TPA + j] = (kappa(a_tau,a_k), kappa(a_tau,b_k), kappa(b_tau,a_k))
+                // for tau = part*TP + j (segments padded to TPA floats for aligned vector reads)
+                const int ct = tid - kColBase; // t-slot staged by this thread
+                if ( ct >= 0 && ct < a.nParts * TP )
                 {
-                    int dt = pRef - tid;
+                    const int cpart = ct / TP, cj = ct - cpart * TP;
+                    const int cdst = cpart * TPA + cj;
+                    int dt = pRef - ct;
                     dt += dt < 0 ? Wc : 0;
                     const int rt = kRef - dt;
-                    const bool ok = tid < Wc && rt >= 0;
+                    const bool ok = ct < Wc && rt >= 0;
                     const bool later = ok && rt > k; // entering after k in this step
                     const float* g = later ? pk + ( rt - c0 ) * rowFloats + 3 * k : pk + i * rowFloats + 3 * ( ok ? rt : 0 );
                     const float m = ok ? 1.f : 0.f;
-                    col[ tid ] = m * g[ 0 ];
-                    col[ kSeqThreads + tid ] = m * ( later ? g[ 1 ] : g[ 2 ] );
-                    col[ 2 * kSeqThreads + tid ] = m * ( later ? g[ 2 ] : g[ 1 ] );
+                    col[ cdst ] = m * g[ 0 ];
+                    col[ kColStride + cdst ] = m * ( later ? g[ 1 ] : g[ 2 ] );
+                    col[ 2 * kColStride + cdst ] = m * ( later ? g[ 2 ] : g[ 1 ] );
                 }
                 // row p: the threads whose t-slots contain p take kappa(k, v)
                 const int jj = p - tBase;
@@ -464,7 +537,7 @@ __global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* _
                 }
             }
         }
-        __syncthreads();
+        ldsBarrier(); // A
         // column p of each entering sample: the thread holding u-slot p takes its t-slots
 #pragma unroll
         for ( int i = 0; i < NEWPER; i++ )
@@ -472,19 +545,30 @@ __global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* _
             const int p = ( cMod + c0 + i ) % Wc;
             if ( gThread && v == p )
             {
-                const float* col = sCol + i * 3 * kSeqThreads + tBase;
+                const float4* col = reinterpret_cast<const float4*>( sCol + i * 3 * kColStride + part * TPA );
 #pragma unroll
-                for ( int j = 0; j < TP; j++ )
+                for ( int j4 = 0; j4 < TPA / 4; j4++ )
                 {
-                    gA[ j ] = col[ j ];
-                    gB[ j ] = col[ kSeqThreads + j ];
-                    gC[ j ] = col[ 2 * kSeqThreads + j ];
+                    const float4 xa = col[ j4 ], xb = col[ kColStride / 4 + j4 ], xc = col[ 2 * kColStride / 4 + j4 ];
+                    const float va[ 4 ] = { xa.x, xa.y, xa.z, xa.w }, vb[ 4 ] = { xb.x, xb.y, xb.z, xb.w },
+                                vc[ 4 ] = { xc.x, xc.y, xc.z, xc.w };
+#pragma unroll
+                    for ( int q4 = 0; q4 < 4; q4++ )
+                    {
+                        if ( 4 * j4 + q4 < TP )
+                        {
+                            gA[ 4 * j4 + q4 ] = va[ q4 ];
+                            gB[ 4 * j4 + q4 ] = vb[ q4 ];
+                            gC[ 4 * j4 + q4 ] = vc[ q4 ];
+                        }
+                    }
                 }
             }
         }
         // 3. projection weights over the window (entering rows complete their f first)
         if ( tid < w2 )
         {
+            const float invW = 1.f / (float)W;
             const int r = sRel - w2 + tid; // the last w2 rows cover every slot once
             int sl = sBase + tid;
             sl -= sl >= w2 ? w2 : 0;
@@ -496,6 +580,7 @@ __global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* _
                 {
                     const int e2 = r - spRel; // 2*entering index + part
                     float acc = pk[ pkRows + e2 ];
+#pragma unroll
                     for ( int w = 0; w < kSeqWaves; w++ )
                         acc += sRedE[ w * 2 * kNewMax + e2 ];
                     f = acc;
@@ -509,15 +594,15 @@ __global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* _
                     b = res - a.eB;
                 else if ( res < -a.eB )
                     b = res + a.eB;
-                b /= (float)W;
-                b /= ( a.wL * sZn[ r >> 1 ] + a.wG );
+                b = b * invW * sInv[ r >> 1 ];
                 beta = b;
                 sC[ r ] += b;
             }
             sB[ sl ] = beta; // sl = 2*complex slot + part
         }
-        __syncthreads();
-        // 4. f_v += sum_t kappa(t, v) beta_t from the register Gram
+        ldsBarrier(); // B
+        // 4. partial f_v += sum_t kappa(t, v) beta_t from the register Gram (summed over the
+        //    t-slot parts at the start of the next step)
         if ( gThread )
         {
             float fa = 0.f, fb = 0.f;
@@ -534,19 +619,20 @@ __global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* _
             sRedF[ 2 * tid ] = fa;
             sRedF[ 2 * tid + 1 ] = fb;
         }
-        __syncthreads();
-        if ( tid < w2 )
-        {
-            const int cv = tid >> 1, pt = tid & 1;
-            float acc = 0.f;
-            for ( int t = 0; t < a.nParts; t++ )
-                acc += sRedF[ 2 * ( t * Wc + cv ) + pt ];
-            sF[ tid ] += acc;
-        }
-        if ( kk + 1 < a.nSteps )
-            pkStore( sPk + ( ( kk + 1 ) & 1 ) * pkF );
-        __syncthreads();
+        // the next step's package (issued two steps ago) must have landed
+        waitVmN( kk + 2 < a.nSteps ? myChunks : 0 );
+        ldsBarrier(); // C
     }
+    if ( tid < w2 )
+    {
+        const int cv = tid >> 1, pt = tid & 1;
+        float acc = 0.f;
+#pragma unroll
+        for ( int t = 0; t < PMAX; t++ )
+            acc += sRedF[ 2 * ( t * Wc + cv ) + pt ];
+        sF[ tid ] += acc;
+    }
+    __syncthreads();
 
     // write back f of the window and c of the active rows; thF += c_j x_j of rows leaving the
     // active set (they take no further projection)
@@ -577,6 +663,16 @@ __global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* _
             acc += sRedF[ ch * L + tid ];
         thF[ u * L + tid ] = acc;
     }
+}
+
+// LDS bytes of apsm_train_seq2<TP, NEWPER> for a window of w2 real rows.
+inline size_t seq2LdsBytes( int newPer, int w2, int tp )
+{
+    const int dpw = ( newPer * 3 * kActMax + 2 * newPer + kSeqThreads + kSeqThreads - 1 ) / kSeqThreads;
+    const int tpa = ( tp + 3 ) & ~3;
+    const size_t floats = 3 * (size_t)dpw * kSeqThreads + w2 + 2 * kSeqThreads + 4 * kActMax + kActMax + kSeqWaves * 2 * kNewMax
+        + 2 * kRedF + (size_t)newPer * 3 * kMaxParts * tpa;
+    return floats * sizeof( float );
 }
 
 } // namespace apsm
