@@ -35,10 +35,10 @@ extern "C" {
 
 /* detectVersion values (cpp/lib/apsm/apsm_versions.cuh:75-89). All four select the same
  * dictionary-streaming kernel here; the names are accepted for drop-in compatibility. */
-#define APSM_DETECT_ORIGINAL 0x10001
-#define APSM_DETECT_OLDFAST 0x10002
-#define APSM_DETECT_SHMEM 0x10006
-#define APSM_DETECT_BALANCED 0x10007
+#define APSM_VERSION_DETECT_ORIGINAL 0x10001
+#define APSM_VERSION_DETECT_OLDFAST 0x10002
+#define APSM_VERSION_DETECT_SHMEM 0x10006
+#define APSM_VERSION_DETECT_BALANCED 0x10007
 
 /* Parameter set. Field meaning, units and defaults follow apsm_parameters
  * (cpp/lib/apsm/apsm_parameters.cuh:68-86) and the factory defaults
@@ -62,6 +62,7 @@ typedef struct apsm_params
     uint8_t training_shuffle;
     uint8_t reserved;
     int32_t detect_version;
+    float linear_kernel_weight; /* w_L; the reference factory sets 1 - gaussian_kernel_weight */
 } apsm_params;
 
 typedef struct apsm_handle apsm_handle;
@@ -120,6 +121,27 @@ void* apsm_stream(apsm_handle* h);
 int apsm_get_state(apsm_handle* h, uint64_t* used_rows, uint32_t* dict_entries, float* variance,
                    float* theta, float* coeffs, uint32_t coeffs_cap);
 
+/* The dictionary's complex basis vectors (the basis of HostTrainingState,
+ * cpp/lib/cuda/cuda_matrix_host.cuh:476-529) in the rx layout [N][dict_entries][2], oldest
+ * first. dict may be NULL to query *dict_entries only. */
+int apsm_get_dictionary(apsm_handle* h, float* dict, uint32_t cap_entries, uint32_t* dict_entries);
+
+/* Replaces ApsmNomaDetector::computeNormRatio (cpp/lib/apsm/apsm_noma_detector.cu:220-319):
+ * (w_L |theta|^2) / (w_G c^T K c) over the live dictionary, for one user of the handle. */
+int apsm_norm_ratio(apsm_handle* h, uint32_t user, float* ratio);
+
+/* Replaces ApsmNomaDetector::train with a validation set (NomaValSet,
+ * cpp/lib/noma/noma_train_history.cuh:99; per-step metrics in apsm_train.cu:147-158): trains
+ * on rx/tx like apsm_train, and after every train step detects the whole training set and the
+ * validation set (rx_val [N][T_val][2], tx_val [U][T_val][2]) and records, per step and user,
+ * history[step][user][0..5] = train EVM, SER, BER, validation EVM, SER, BER with the rules of
+ * cpp/lib/noma/noma_train_history.cu:105-236 for a constellation of bits_per_symbol bits and
+ * the given modulation scale. *num_steps receives the number of steps (<= history_cap).
+ * Needs train_passes == 1 (APSM_ERR_UNSUPPORTED otherwise). *ms: device time of the loop. */
+int apsm_train_history(apsm_handle* h, const float* rx, const float* tx, uint32_t T, const float* rx_val,
+                       const float* tx_val, uint32_t T_val, uint32_t bits_per_symbol, float modulation_scale,
+                       float* history, uint32_t history_cap, uint32_t* num_steps, float* ms);
+
 /* LLS detector: replaces LlsNomaDetector (cpp/lib/noma/lls_detector.cu:97-218).
  * Normal equations accumulated on the device in FP32, solved by an FP64 Cholesky on the
  * device; detect is est = A w. lls_get_weights replaces getWeightMat() (2N floats). */
@@ -131,6 +153,38 @@ int lls_train_dev(lls_handle* h, const float* d_rx, const float* d_tx, uint32_t 
 int lls_detect_dev(lls_handle* h, const float* d_rx, uint32_t M, float* d_est);
 int lls_get_weights(lls_handle* h, float* w);
 int lls_synchronize(lls_handle* h);
+
+/* LlsNomaDetector(numAntennas, std::shared_ptr<CudaStream> extStream)
+ * (cpp/lib/noma/lls_detector.cu:97-107): the handle enqueues on the caller's stream and
+ * does not destroy it. */
+int lls_create_on_stream(uint32_t num_antennas, void* stream, lls_handle** out);
+
+/* trainLowLevel (cpp/lib/noma/lls_detector.cu:157-179): least squares over a general
+ * column-major device matrix A (num_samples x num_input_dims, leading dimension
+ * num_samples) and device target y (num_samples). num_output_dims must be 1 (the only value
+ * the reference passes); num_input_dims <= 128. A and y are not modified. */
+int lls_train_lowlevel(lls_handle* h, const float* d_A, const float* d_y, int32_t num_input_dims,
+                       int32_t num_output_dims, int32_t num_samples);
+
+/* detectLowLevel (cpp/lib/noma/lls_detector.cu:207-218): est = alpha * A w + beta * est on
+ * the handle stream (est is not read when beta == 0). */
+int lls_detect_lowlevel(lls_handle* h, const float* d_A, float* d_est, int32_t num_input_dims,
+                        int32_t num_output_dims, int32_t num_samples, float alpha, float beta);
+
+/* computeSignalPower (cpp/lib/noma/lls_detector.cu:236-242): sqrt(sum d^2 / n) of a device
+ * array; synchronises the handle stream. */
+int lls_signal_power(lls_handle* h, const float* d_data, uint32_t num_values, float* out);
+
+/* getWeightMat() (cpp/lib/noma/lls_detector.cuh:92): device pointer of the solved weights. */
+float* lls_weights_dev(lls_handle* h);
+
+/* The handle's stream (hipStream_t), for callers that order their own work after it. */
+void* lls_stream(lls_handle* h);
+
+/* CudaStream (cpp/lib/cuda/cuda_stream.cuh:67-96): a HIP stream of default priority. */
+int apsm_stream_create(void** out);
+int apsm_stream_destroy(void* stream);
+int apsm_stream_synchronize(void* stream);
 
 /* Number of visible HIP devices (0 when no GPU is present). Never fails. */
 int apsm_device_count(void);

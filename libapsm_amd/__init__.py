@@ -20,6 +20,7 @@ from .detector import (  # noqa: F401
     LlsNomaDetectorFactory,
     NomaDetector,
     NomaDetectorFactory,
+    NomaTrainHistory,
     apsm_get_version_string,
     apsm_parameters,
 )

@@ -33,6 +33,7 @@ class ApsmParamsC(ctypes.Structure):
         ("training_shuffle", ctypes.c_uint8),
         ("reserved", ctypes.c_uint8),
         ("detect_version", ctypes.c_int32),
+        ("linear_kernel_weight", ctypes.c_float),
     ]
 
 
@@ -64,6 +65,20 @@ SIGNATURES = {
     "lls_detect_dev": (ctypes.c_int, [_vp, _vp, _u32, _vp]),
     "lls_get_weights": (ctypes.c_int, [_vp, _vp]),
     "lls_synchronize": (ctypes.c_int, [_vp]),
+    "apsm_get_dictionary": (ctypes.c_int, [_vp, _vp, _u32, _P(ctypes.c_uint32)]),
+    "apsm_norm_ratio": (ctypes.c_int, [_vp, _u32, _f32p]),
+    "apsm_train_history": (ctypes.c_int, [_vp, _vp, _vp, _u32, _vp, _vp, _u32, _u32, ctypes.c_float, _vp, _u32,
+                                          _P(ctypes.c_uint32), _f32p]),
+    "lls_create_on_stream": (ctypes.c_int, [_u32, _vp, _P(_vp)]),
+    "lls_train_lowlevel": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "lls_detect_lowlevel": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_float, ctypes.c_float]),
+    "lls_signal_power": (ctypes.c_int, [_vp, _vp, _u32, _f32p]),
+    "lls_weights_dev": (_vp, [_vp]),
+    "lls_stream": (_vp, [_vp]),
+    "apsm_stream_create": (ctypes.c_int, [_P(_vp)]),
+    "apsm_stream_destroy": (ctypes.c_int, [_vp]),
+    "apsm_stream_synchronize": (ctypes.c_int, [_vp]),
 }
 
 _lib = None

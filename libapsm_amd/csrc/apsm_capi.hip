@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <random>
+#include <vector>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -71,6 +72,10 @@ struct DevBuf
 {
     T* ptr = nullptr;
     size_t cap = 0;
+    DevBuf() = default;
+    DevBuf( const DevBuf& ) = delete;
+    DevBuf& operator=( const DevBuf& ) = delete;
+    ~DevBuf() { release(); }
     void ensure( size_t n )
     {
         if ( n <= cap )
@@ -166,18 +171,40 @@ int gridFor( int64_t n, int threads ) { return (int)( ( n + threads - 1 ) / thre
 // ---------------------------------------------------------------------------------------
 // LLS
 // ---------------------------------------------------------------------------------------
+constexpr int kLlsMaxDims = 128; // unknowns of one least-squares solve (the FP64 Cholesky is LDS-resident)
+
 struct lls_handle
 {
     int N = 0, L = 0;
     hipStream_t stream = nullptr;
+    bool ownStream = true; // false when the caller lent its stream (lls_create_on_stream)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf<float> w, part, rx, tx, est;
-    DevBuf<double> gram;
+    DevBuf<double> gram, sums;
     DevBuf<int> status;
+    int nW = 0; // unknowns of the last solve (2N for the embedding, num_input_dims for the low-level call)
     bool trained = false;
 };
 
 namespace {
+
+// FP64 sum of the nPart partial normal equations of n unknowns, then the FP64 Cholesky solve
+// into h->w (first n entries) and h->status.
+void llsReduceSolve( lls_handle* h, int nPart, int n )
+{
+    const int E = n * ( n + 1 );
+    h->nW = n;
+    h->gram.ensure( (size_t)E );
+    hipLaunchKernelGGL( apsm::lls_reduce_partials, dim3( gridFor( E, 256 ) ), dim3( 256 ), 0, h->stream, h->part.ptr, nPart, E,
+                        h->gram.ptr );
+    HIP_OR_THROW( hipGetLastError() );
+    const size_t ldsSolve = (size_t)E * sizeof( double );
+    if ( ldsSolve > 64 * 1024 )
+        HIP_OR_THROW( hipFuncSetAttribute( (const void*)apsm::lls_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsSolve ) );
+    hipLaunchKernelGGL( apsm::lls_solve, dim3( 1 ), dim3( apsm::kSolveThreads ), ldsSolve, h->stream, h->gram.ptr, n, h->w.ptr,
+                        h->status.ptr );
+    HIP_OR_THROW( hipGetLastError() );
+}
 
 void llsTrainDev( lls_handle* h, const float* d_rx, const float* d_tx, int64_t T )
 {
@@ -187,20 +214,29 @@ void llsTrainDev( lls_handle* h, const float* d_rx, const float* d_tx, int64_t T
     const int64_t chunk = std::max<int64_t>( 1, (int64_t)( apsm::kLlsLdsBudget / ( 2 * ( L + 1 ) * sizeof( float ) ) ) );
     const int nPart = (int)( ( T + chunk - 1 ) / chunk );
     h->part.ensure( (size_t)nPart * E );
-    h->gram.ensure( (size_t)E );
     const size_t lds = 2 * (size_t)chunk * ( L + 1 ) * sizeof( float );
     hipLaunchKernelGGL( apsm::lls_normal_partial, dim3( nPart ), dim3( apsm::kLlsThreads ), lds, h->stream, d_rx, d_tx, h->N,
                         T, chunk, h->part.ptr );
     HIP_OR_THROW( hipGetLastError() );
-    hipLaunchKernelGGL( apsm::lls_reduce_partials, dim3( gridFor( E, 256 ) ), dim3( 256 ), 0, h->stream, h->part.ptr, nPart, E,
-                        h->gram.ptr );
+    llsReduceSolve( h, nPart, L );
+}
+
+// General column-major A (m x n, ld m), target y (m): trainLowLevel's operands.
+void llsTrainColMajor( lls_handle* h, const float* d_A, const float* d_y, int n, int64_t m )
+{
+    ARG_CHECK( m > 0 && n > 0, "LLS train needs at least one row and one column" );
+    ARG_CHECK( n <= kLlsMaxDims, "LLS supports at most 128 input dimensions" );
+    const int E = n * ( n + 1 );
+    const int64_t chunk = std::max<int64_t>( 1, (int64_t)( apsm::kLlsLdsBudget / ( ( n + 1 ) * sizeof( float ) ) ) );
+    const int64_t nPart64 = ( m + chunk - 1 ) / chunk;
+    ARG_CHECK( nPart64 < ( 1 << 30 ), "too many rows" );
+    const int nPart = (int)nPart64;
+    h->part.ensure( (size_t)nPart * E );
+    const size_t lds = (size_t)chunk * ( n + 1 ) * sizeof( float );
+    hipLaunchKernelGGL( apsm::lls_normal_colmajor, dim3( nPart ), dim3( apsm::kLlsThreads ), lds, h->stream, d_A, d_y, n, m, chunk,
+                        h->part.ptr );
     HIP_OR_THROW( hipGetLastError() );
-    const size_t ldsSolve = (size_t)E * sizeof( double );
-    if ( ldsSolve > 64 * 1024 )
-        HIP_OR_THROW( hipFuncSetAttribute( (const void*)apsm::lls_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsSolve ) );
-    hipLaunchKernelGGL( apsm::lls_solve, dim3( 1 ), dim3( apsm::kSolveThreads ), ldsSolve, h->stream, h->gram.ptr, L, h->w.ptr,
-                        h->status.ptr );
-    HIP_OR_THROW( hipGetLastError() );
+    llsReduceSolve( h, nPart, n );
 }
 
 void llsCheckStatus( lls_handle* h )
@@ -270,17 +306,20 @@ void validateParams( const apsm_params* p, uint32_t N, uint32_t U )
     ARG_CHECK( 2 * p->window_size <= 1024, "windowSize must be at most 512" );
     ARG_CHECK( p->sample_step >= 2 && p->sample_step % 2 == 0, "sampleStep must be a positive even number" );
     ARG_CHECK( p->train_passes >= 1, "trainPasses must be at least 1" );
-    ARG_CHECK( p->gaussian_kernel_weight >= 0.f && p->gaussian_kernel_weight < 1.f, "gaussianKernelWeight must be in [0, 1)" );
+    ARG_CHECK( p->gaussian_kernel_weight >= 0.f && p->gaussian_kernel_weight <= 1.f, "gaussianKernelWeight must be in [0, 1]" );
+    ARG_CHECK( p->linear_kernel_weight >= 0.f, "linearKernelWeight must be non-negative" );
+    ARG_CHECK( p->linear_kernel_weight > 0.f || p->gaussian_kernel_weight > 0.f, "kernel weights must not both be zero" );
+    ARG_CHECK( !p->lls_initialization || p->linear_kernel_weight > 0.f, "llsInitialization needs a positive linear kernel weight" );
     ARG_CHECK( p->gaussian_kernel_variance >= 0.f, "gaussianKernelVariance must be non-negative" );
     ARG_CHECK( p->eB >= 0.f, "eB must be non-negative" );
     if ( p->norm_constraint > 0.f )
         throw ApsmError { APSM_ERR_UNSUPPORTED, "normConstraint > 0 is not implemented on the device path" };
     switch ( p->detect_version )
     {
-    case APSM_DETECT_ORIGINAL:
-    case APSM_DETECT_OLDFAST:
-    case APSM_DETECT_SHMEM:
-    case APSM_DETECT_BALANCED:
+    case APSM_VERSION_DETECT_ORIGINAL:
+    case APSM_VERSION_DETECT_OLDFAST:
+    case APSM_VERSION_DETECT_SHMEM:
+    case APSM_VERSION_DETECT_BALANCED:
         break;
     default:
         throw ApsmError { APSM_ERR_ARG, "unknown detect version" };
@@ -487,17 +526,13 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
     }
 }
 
-void trainDev( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, const float* hostRx )
+// Kernel variance (configured value scaled by N^2, or the heuristic on the call's pilots) and
+// the LLS initialisation of theta, both from the whole call (apsm_noma_detector.cu:96-105).
+void trainPrepare( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, const float* hostRx,
+                   const std::vector<int32_t>* perm )
 {
     const apsm_params& p = h->p;
-    ARG_CHECK( T > 0, "train needs at least one pilot" );
-    ARG_CHECK( p.train_passes == 1 || h->used == 0, "trainPasses > 1 requires a fresh state (resetState)" );
     const int N = h->N, L = h->L, U = h->U;
-
-    std::vector<int32_t> perm;
-    if ( p.training_shuffle )
-        perm = shufflePermutation( T );
-
     if ( !h->varSet )
     {
         if ( p.gaussian_kernel_variance != 0.f )
@@ -513,11 +548,11 @@ void trainDev( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, 
                 HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
                 src = tmp.data();
             }
-            h->var = varianceHeuristic( src, N, T, p.training_shuffle ? &perm : nullptr );
+            h->var = varianceHeuristic( src, N, T, perm );
         }
         h->varSet = true;
     }
-    const float wG = p.gaussian_kernel_weight, wL = 1.f - wG;
+    const float wL = p.linear_kernel_weight;
 
     if ( p.lls_initialization && !h->anyTrain )
     {
@@ -541,6 +576,24 @@ void trainDev( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, 
         }
     }
     h->anyTrain = true;
+}
+
+// One train call. prepared: the caller already ran trainPrepare over the whole pilot set and
+// the rows arrive in final order (no shuffle here).
+void trainDev( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, const float* hostRx, bool prepared = false )
+{
+    const apsm_params& p = h->p;
+    ARG_CHECK( T > 0, "train needs at least one pilot" );
+    ARG_CHECK( p.train_passes == 1 || h->used == 0, "trainPasses > 1 requires a fresh state (resetState)" );
+    const int N = h->N, L = h->L, U = h->U;
+
+    std::vector<int32_t> perm;
+    const bool shuffle = p.training_shuffle && !prepared;
+    if ( shuffle )
+        perm = shufflePermutation( T );
+    if ( !prepared )
+        trainPrepare( h, d_rx, d_tx, T, hostRx, shuffle ? &perm : nullptr );
+    const float wG = p.gaussian_kernel_weight, wL = p.linear_kernel_weight;
 
     // New stream buffer: the dictionary rows still live (at most Dc complex entries) + this call.
     const int64_t usedC = (int64_t)( h->used / 2 );
@@ -564,7 +617,7 @@ void trainDev( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, 
                                         2 * nOld * sizeof( float ), U, hipMemcpyDeviceToDevice, h->stream ) );
     }
     const int32_t* dPerm = nullptr;
-    if ( p.training_shuffle )
+    if ( shuffle )
     {
         h->perm.ensure( (size_t)T );
         HIP_OR_THROW( hipMemcpyAsync( h->perm.ptr, perm.data(), T * sizeof( int32_t ), hipMemcpyHostToDevice, h->stream ) );
@@ -627,7 +680,7 @@ template <int NMAX, int UMAX>
 void launchDetect( apsm_handle* h, const float* d_rx, int64_t M, float* d_est, const float* Z, const float* Zn, const float* C,
                    int64_t ldC, const float* theta, int64_t G, int U )
 {
-    const float wG = h->p.gaussian_kernel_weight, wL = 1.f - wG;
+    const float wG = h->p.gaussian_kernel_weight, wL = h->p.linear_kernel_weight;
     const float g2 = 0.5f / h->var * kLog2e;
     hipLaunchKernelGGL( ( apsm::apsm_detect_kernel<NMAX, UMAX> ), dim3( gridFor( M, apsm::kDetThreads ) ), dim3( apsm::kDetThreads ),
                         0, h->stream, d_rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, d_est, h->N );
@@ -714,7 +767,8 @@ void apsm_params_default( apsm_params* p )
     p->start_with_full_window = 0;
     p->lls_initialization = 0;
     p->training_shuffle = 0;
-    p->detect_version = APSM_DETECT_BALANCED;
+    p->detect_version = APSM_VERSION_DETECT_BALANCED;
+    p->linear_kernel_weight = 0.5f;
 }
 
 const char* apsm_last_error( void ) { return g_lastError.c_str(); }
@@ -908,8 +962,176 @@ int apsm_get_state( apsm_handle* h, uint64_t* used_rows, uint32_t* dict_entries,
     } );
 }
 
+int apsm_get_dictionary( apsm_handle* h, float* dict, uint32_t cap_entries, uint32_t* dict_entries )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h != nullptr, "null handle" );
+        HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+        const int64_t usedC = (int64_t)( h->used / 2 );
+        const int64_t lo = std::max<int64_t>( 0, usedC - h->Hc );
+        const int64_t G = usedC - lo;
+        if ( dict_entries )
+            *dict_entries = (uint32_t)G;
+        if ( !dict || G == 0 )
+            return;
+        ARG_CHECK( cap_entries >= G, "dictionary buffer too small" );
+        const StreamBuf& b = h->buf[ h->cur ];
+        std::vector<float> z( (size_t)G * h->L );
+        HIP_OR_THROW( hipMemcpy( z.data(), b.Z.ptr + ( lo - h->bufBaseC ) * h->L, z.size() * sizeof( float ), hipMemcpyDeviceToHost ) );
+        for ( int64_t k = 0; k < G; k++ )
+            for ( int n = 0; n < h->N; n++ )
+            {
+                dict[ ( (size_t)n * G + k ) * 2 ] = z[ k * h->L + n ];
+                dict[ ( (size_t)n * G + k ) * 2 + 1 ] = z[ k * h->L + h->N + n ];
+            }
+    } );
+}
+
+int apsm_norm_ratio( apsm_handle* h, uint32_t user, float* ratio )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && ratio, "null argument" );
+        ARG_CHECK( user < (uint32_t)h->U, "user index out of range" );
+        const int64_t usedC = (int64_t)( h->used / 2 );
+        const int64_t lo = std::max<int64_t>( 0, usedC - h->Hc );
+        const int64_t G = usedC - lo;
+        const float wG = h->p.gaussian_kernel_weight, wL = h->p.linear_kernel_weight;
+        double gauss = 0.0;
+        if ( G > 0 )
+        {
+            const StreamBuf& b = h->buf[ h->cur ];
+            const int64_t off = lo - h->bufBaseC;
+            const int nb = gridFor( G, apsm::kQuadThreads );
+            DevBuf<double> part;
+            part.ensure( (size_t)nb );
+            const size_t lds = sizeof( float ) * ( (size_t)apsm::kQuadTile * h->L + 3 * apsm::kQuadTile );
+            hipLaunchKernelGGL( apsm::apsm_quad_form_partial, dim3( nb ), dim3( apsm::kQuadThreads ), lds, h->stream,
+                                b.Z.ptr + off * h->L, b.Zn.ptr + off, b.C.ptr + (size_t)user * 2 * b.capK + 2 * off, G, h->N,
+                                0.5f / h->var * kLog2e, part.ptr );
+            HIP_OR_THROW( hipGetLastError() );
+            std::vector<double> hp( nb );
+            HIP_OR_THROW( hipMemcpyAsync( hp.data(), part.ptr, nb * sizeof( double ), hipMemcpyDeviceToHost, h->stream ) );
+            HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+            part.release();
+            for ( double v : hp )
+                gauss += v;
+        }
+        std::vector<float> th( h->L );
+        HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+        HIP_OR_THROW( hipMemcpy( th.data(), h->theta.ptr + (size_t)user * h->L, h->L * sizeof( float ), hipMemcpyDeviceToHost ) );
+        double lin = 0.0;
+        for ( float v : th )
+            lin += (double)v * v;
+        // linear norm / Gaussian norm (apsm_noma_detector.cu:300-318)
+        *ratio = (float)( ( wL * lin ) / ( wG * gauss ) );
+    } );
+}
+
+int apsm_train_history( apsm_handle* h, const float* rx, const float* tx, uint32_t T, const float* rx_val, const float* tx_val,
+                        uint32_t T_val, uint32_t bits_per_symbol, float modulation_scale, float* history, uint32_t history_cap,
+                        uint32_t* num_steps, float* ms )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && rx && tx && rx_val && tx_val && history && num_steps, "null argument" );
+        ARG_CHECK( T > 0 && T_val > 0, "training and validation sets must not be empty" );
+        ARG_CHECK( bits_per_symbol >= 1 && bits_per_symbol <= 16 && ( bits_per_symbol == 1 || bits_per_symbol % 2 == 0 ),
+                   "bitsPerSymbol must be 1 or even" );
+        ARG_CHECK( modulation_scale > 0.f, "modulation scale must be positive" );
+        const apsm_params& p = h->p;
+        if ( p.train_passes != 1 )
+            throw ApsmError { APSM_ERR_UNSUPPORTED, "a training history needs trainPasses == 1" };
+        const int N = h->N, U = h->U;
+        const int64_t q2 = p.sample_step / 2; // complex samples per step
+        const int64_t firstC = ( p.start_with_full_window && h->used == 0 ) ? h->w2 / 2 : q2;
+        const int64_t steps = T >= firstC ? 1 + ( T - firstC ) / q2 : 0;
+        ARG_CHECK( (uint64_t)steps <= history_cap, "history buffer too small" );
+
+        // pilots in training order (the reference shuffles before anything else, apsm_noma_detector.cu:102-106)
+        std::vector<float> hrx( rx, rx + (size_t)N * T * 2 ), htx( tx, tx + (size_t)U * T * 2 );
+        if ( p.training_shuffle )
+        {
+            const std::vector<int32_t> perm = shufflePermutation( T );
+            for ( int n = 0; n < N; n++ )
+                for ( int64_t k = 0; k < T; k++ )
+                {
+                    hrx[ ( n * T + k ) * 2 ] = rx[ ( n * T + perm[ k ] ) * 2 ];
+                    hrx[ ( n * T + k ) * 2 + 1 ] = rx[ ( n * T + perm[ k ] ) * 2 + 1 ];
+                }
+            for ( int u = 0; u < U; u++ )
+                for ( int64_t k = 0; k < T; k++ )
+                {
+                    htx[ ( u * T + k ) * 2 ] = tx[ ( u * T + perm[ k ] ) * 2 ];
+                    htx[ ( u * T + k ) * 2 + 1 ] = tx[ ( u * T + perm[ k ] ) * 2 + 1 ];
+                }
+        }
+        DevBuf<float> dRx, dTx, dRv, dTv, dEst, dChunkRx, dChunkTx, dHist;
+        const int64_t Tmax = std::max<int64_t>( T, T_val );
+        dRx.ensure( hrx.size() );
+        dTx.ensure( htx.size() );
+        dRv.ensure( (size_t)N * T_val * 2 );
+        dTv.ensure( (size_t)U * T_val * 2 );
+        dEst.ensure( (size_t)U * Tmax * 2 );
+        dChunkRx.ensure( (size_t)N * std::max( firstC, q2 ) * 2 );
+        dChunkTx.ensure( (size_t)U * std::max( firstC, q2 ) * 2 );
+        dHist.ensure( (size_t)std::max<int64_t>( steps, 1 ) * U * 6 );
+        HIP_OR_THROW( hipMemcpyAsync( dRx.ptr, hrx.data(), hrx.size() * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
+        HIP_OR_THROW( hipMemcpyAsync( dTx.ptr, htx.data(), htx.size() * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
+        HIP_OR_THROW( hipMemcpyAsync( dRv.ptr, rx_val, (size_t)N * T_val * 2 * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
+        HIP_OR_THROW( hipMemcpyAsync( dTv.ptr, tx_val, (size_t)U * T_val * 2 * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
+        trainPrepare( h, dRx.ptr, dTx.ptr, T, hrx.data(), nullptr );
+
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIP_OR_THROW( hipEventCreate( &e0 ) );
+        HIP_OR_THROW( hipEventCreate( &e1 ) );
+        HIP_OR_THROW( hipEventRecord( e0, h->stream ) );
+        int64_t c0 = 0;
+        for ( int64_t st = 0; st < steps; st++ )
+        {
+            const int64_t nC = st == 0 ? firstC : q2;
+            // one step's pilots as a contiguous [N][nC][2] / [U][nC][2] call
+            HIP_OR_THROW( hipMemcpy2DAsync( dChunkRx.ptr, nC * 2 * sizeof( float ), dRx.ptr + c0 * 2, T * 2 * sizeof( float ),
+                                            nC * 2 * sizeof( float ), N, hipMemcpyDeviceToDevice, h->stream ) );
+            HIP_OR_THROW( hipMemcpy2DAsync( dChunkTx.ptr, nC * 2 * sizeof( float ), dTx.ptr + c0 * 2, T * 2 * sizeof( float ),
+                                            nC * 2 * sizeof( float ), U, hipMemcpyDeviceToDevice, h->stream ) );
+            trainDev( h, dChunkRx.ptr, dChunkTx.ptr, nC, nullptr, true );
+            c0 += nC;
+            // detect the whole training set and the validation set with the state after this step
+            // (apsm_train.cu:147-158)
+            detectDev( h, dRx.ptr, T, dEst.ptr );
+            hipLaunchKernelGGL( apsm::apsm_history_metrics, dim3( U ), dim3( apsm::kMetricThreads ), 0, h->stream, dTx.ptr,
+                                (int64_t)T, dEst.ptr, (int64_t)T, (int64_t)T, (int)bits_per_symbol, modulation_scale,
+                                dHist.ptr + (size_t)st * U * 6, 6 );
+            HIP_OR_THROW( hipGetLastError() );
+            detectDev( h, dRv.ptr, T_val, dEst.ptr );
+            hipLaunchKernelGGL( apsm::apsm_history_metrics, dim3( U ), dim3( apsm::kMetricThreads ), 0, h->stream, dTv.ptr,
+                                (int64_t)T_val, dEst.ptr, (int64_t)T_val, (int64_t)T_val, (int)bits_per_symbol,
+                                modulation_scale, dHist.ptr + (size_t)st * U * 6 + 3, 6 );
+            HIP_OR_THROW( hipGetLastError() );
+        }
+        HIP_OR_THROW( hipEventRecord( e1, h->stream ) );
+        const float t = elapsedMs( e0, e1 );
+        (void)hipEventDestroy( e0 );
+        (void)hipEventDestroy( e1 );
+        if ( steps > 0 )
+            HIP_OR_THROW( hipMemcpyAsync( history, dHist.ptr, (size_t)steps * U * 6 * sizeof( float ), hipMemcpyDeviceToHost,
+                                          h->stream ) );
+        HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+        *num_steps = (uint32_t)steps;
+        if ( ms )
+            *ms = t;
+        dRx.release();
+        dTx.release();
+        dRv.release();
+        dTv.release();
+        dEst.release();
+        dChunkRx.release();
+        dChunkTx.release();
+        dHist.release();
+    } );
+}
+
 // ---- LLS ----
-int lls_create( uint32_t num_antennas, lls_handle** out )
+static int llsCreate( uint32_t num_antennas, void* stream, lls_handle** out )
 {
     return guarded( [ & ] {
         ARG_CHECK( out != nullptr, "null output handle" );
@@ -920,12 +1142,18 @@ int lls_create( uint32_t num_antennas, lls_handle** out )
         {
             h->N = (int)num_antennas;
             h->L = 2 * h->N;
-            HIP_OR_THROW( hipStreamCreateWithFlags( &h->stream, hipStreamNonBlocking ) );
+            if ( stream )
+            {
+                h->stream = (hipStream_t)stream;
+                h->ownStream = false;
+            }
+            else
+                HIP_OR_THROW( hipStreamCreateWithFlags( &h->stream, hipStreamNonBlocking ) );
             HIP_OR_THROW( hipEventCreate( &h->ev0 ) );
             HIP_OR_THROW( hipEventCreate( &h->ev1 ) );
-            h->w.ensure( (size_t)h->L );
+            h->w.ensure( (size_t)kLlsMaxDims );
             h->status.ensure( 1 );
-            HIP_OR_THROW( hipMemsetAsync( h->w.ptr, 0, h->L * sizeof( float ), h->stream ) );
+            HIP_OR_THROW( hipMemsetAsync( h->w.ptr, 0, kLlsMaxDims * sizeof( float ), h->stream ) );
             HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
         }
         catch ( ... )
@@ -935,6 +1163,15 @@ int lls_create( uint32_t num_antennas, lls_handle** out )
         }
         *out = h;
     } );
+}
+
+int lls_create( uint32_t num_antennas, lls_handle** out ) { return llsCreate( num_antennas, nullptr, out ); }
+
+int lls_create_on_stream( uint32_t num_antennas, void* stream, lls_handle** out )
+{
+    if ( !stream )
+        return guarded( [] { throw ApsmError { APSM_ERR_ARG, "null stream" }; } );
+    return llsCreate( num_antennas, stream, out );
 }
 
 void lls_destroy( lls_handle* h )
@@ -950,11 +1187,12 @@ void lls_destroy( lls_handle* h )
     h->tx.release();
     h->est.release();
     h->status.release();
+    h->sums.release();
     if ( h->ev0 )
         (void)hipEventDestroy( h->ev0 );
     if ( h->ev1 )
         (void)hipEventDestroy( h->ev1 );
-    if ( h->stream )
+    if ( h->stream && h->ownStream )
         (void)hipStreamDestroy( h->stream );
     delete h;
 }
@@ -994,6 +1232,7 @@ int lls_detect_dev( lls_handle* h, const float* d_rx, uint32_t M, float* d_est )
     return guarded( [ & ] {
         ARG_CHECK( h && d_rx && d_est, "null argument" );
         ARG_CHECK( M > 0, "detect needs at least one sample" );
+        ARG_CHECK( h->nW == 0 || h->nW == h->L, "weights come from a low-level solve of another size" );
         hipLaunchKernelGGL( apsm::lls_apply, dim3( gridFor( M, 256 ) ), dim3( 256 ), 0, h->stream, d_rx, h->N, (int64_t)M, h->w.ptr,
                             d_est );
         HIP_OR_THROW( hipGetLastError() );
@@ -1005,6 +1244,7 @@ int lls_detect( lls_handle* h, const float* rx, uint32_t M, float* est, float* m
     return guarded( [ & ] {
         ARG_CHECK( h && rx && est, "null argument" );
         ARG_CHECK( M > 0, "detect needs at least one sample" );
+        ARG_CHECK( h->nW == 0 || h->nW == h->L, "weights come from a low-level solve of another size" );
         const size_t nrx = (size_t)h->N * M * 2, nest = (size_t)M * 2;
         h->rx.ensure( nrx );
         h->est.ensure( nest );
@@ -1022,12 +1262,87 @@ int lls_detect( lls_handle* h, const float* rx, uint32_t M, float* est, float* m
     } );
 }
 
+int lls_train_lowlevel( lls_handle* h, const float* d_A, const float* d_y, int32_t num_input_dims, int32_t num_output_dims,
+                        int32_t num_samples )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && d_A && d_y, "null argument" );
+        ARG_CHECK( num_output_dims == 1, "LLS solves for one output dimension" );
+        llsTrainColMajor( h, d_A, d_y, num_input_dims, num_samples );
+        llsCheckStatus( h );
+        h->trained = true;
+    } );
+}
+
+int lls_detect_lowlevel( lls_handle* h, const float* d_A, float* d_est, int32_t num_input_dims, int32_t num_output_dims,
+                         int32_t num_samples, float alpha, float beta )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && d_A && d_est, "null argument" );
+        ARG_CHECK( num_output_dims == 1, "LLS solves for one output dimension" );
+        ARG_CHECK( num_input_dims > 0 && num_input_dims <= kLlsMaxDims && num_samples > 0, "invalid matrix shape" );
+        hipLaunchKernelGGL( apsm::lls_gemv_colmajor, dim3( gridFor( num_samples, 256 ) ), dim3( 256 ), 0, h->stream, d_A,
+                            num_input_dims, (int64_t)num_samples, h->w.ptr, alpha, beta, d_est );
+        HIP_OR_THROW( hipGetLastError() );
+    } );
+}
+
+int lls_signal_power( lls_handle* h, const float* d_data, uint32_t num_values, float* out )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && d_data && out, "null argument" );
+        ARG_CHECK( num_values > 0, "no values" );
+        const int nb = std::min( 1024, gridFor( num_values, 256 ) );
+        h->sums.ensure( (size_t)nb );
+        hipLaunchKernelGGL( apsm::sum_squares_partial, dim3( nb ), dim3( 256 ), 0, h->stream, d_data, (int64_t)num_values,
+                            h->sums.ptr );
+        HIP_OR_THROW( hipGetLastError() );
+        std::vector<double> part( nb );
+        HIP_OR_THROW( hipMemcpyAsync( part.data(), h->sums.ptr, nb * sizeof( double ), hipMemcpyDeviceToHost, h->stream ) );
+        HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+        double s = 0.0;
+        for ( double v : part )
+            s += v;
+        *out = (float)std::sqrt( s / num_values );
+    } );
+}
+
+float* lls_weights_dev( lls_handle* h ) { return h ? h->w.ptr : nullptr; }
+
+void* lls_stream( lls_handle* h ) { return h ? (void*)h->stream : nullptr; }
+
+int apsm_stream_create( void** out )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( out != nullptr, "null output" );
+        hipStream_t s = nullptr;
+        HIP_OR_THROW( hipStreamCreateWithFlags( &s, hipStreamNonBlocking ) );
+        *out = (void*)s;
+    } );
+}
+
+int apsm_stream_destroy( void* stream )
+{
+    return guarded( [ & ] {
+        if ( stream )
+            HIP_OR_THROW( hipStreamDestroy( (hipStream_t)stream ) );
+    } );
+}
+
+int apsm_stream_synchronize( void* stream )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( stream != nullptr, "null stream" );
+        HIP_OR_THROW( hipStreamSynchronize( (hipStream_t)stream ) );
+    } );
+}
+
 int lls_get_weights( lls_handle* h, float* w )
 {
     return guarded( [ & ] {
         ARG_CHECK( h && w, "null argument" );
         HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
-        HIP_OR_THROW( hipMemcpy( w, h->w.ptr, h->L * sizeof( float ), hipMemcpyDeviceToHost ) );
+        HIP_OR_THROW( hipMemcpy( w, h->w.ptr, ( h->nW ? h->nW : h->L ) * sizeof( float ), hipMemcpyDeviceToHost ) );
     } );
 }
 

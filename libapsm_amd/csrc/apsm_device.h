@@ -644,12 +644,205 @@ __global__ void lls_apply( const float* __restrict__ rx, int N, int64_t M, const
     reinterpret_cast<float2*>( est )[ m ] = make_float2( ea, eb );
 }
 
+// Low-level LLS over a general column-major real matrix A (m x n, leading dimension m) and
+// target y (m), as trainLowLevel / detectLowLevel take them (lls_detector.cu:157-179,207-218).
+// part[b][i*(n+1) + j]: sum over rows of chunk b of A_ri A_rj (j < n) and A_ri y_r (j == n).
+__global__ void __launch_bounds__( kLlsThreads ) lls_normal_colmajor( const float* __restrict__ A, const float* __restrict__ y,
+                                                                    int n, int64_t m, int64_t chunk, float* __restrict__ part )
+{
+    const int ld = n + 1;
+    const int E = n * ld;
+    extern __shared__ float sA[]; // [chunk][n+1]
+    const int64_t r0 = (int64_t)blockIdx.x * chunk;
+    const int nt = (int)( ( r0 + chunk ) < m ? chunk : m - r0 );
+    for ( int i = threadIdx.x; i < nt * ld; i += kLlsThreads )
+    {
+        const int c = i / nt, t = i % nt; // consecutive threads read consecutive rows of one column
+        sA[ t * ld + c ] = c < n ? A[ (int64_t)c * m + r0 + t ] : y[ r0 + t ];
+    }
+    __syncthreads();
+    for ( int e = threadIdx.x; e < E; e += kLlsThreads )
+    {
+        const int i = e / ld, j = e % ld;
+        float s = 0.f;
+        for ( int t = 0; t < nt; t++ )
+            s += sA[ t * ld + i ] * sA[ t * ld + j ];
+        part[ (int64_t)blockIdx.x * E + e ] = s;
+    }
+}
+
+// est[r] = alpha * sum_i A[i*m + r] w[i] + beta * est[r]  (cublasSgemm with one column).
+__global__ void lls_gemv_colmajor( const float* __restrict__ A, int n, int64_t m, const float* __restrict__ w, float alpha,
+                                   float beta, float* __restrict__ est )
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ( r >= m )
+        return;
+    float s = 0.f;
+    for ( int i = 0; i < n; i++ )
+        s += A[ (int64_t)i * m + r ] * w[ i ];
+    est[ r ] = beta == 0.f ? alpha * s : alpha * s + beta * est[ r ];
+}
+
+// out[b] = sum of squares over a grid-stride share of data (FP64 partials per workgroup).
+__global__ void __launch_bounds__( 256 ) sum_squares_partial( const float* __restrict__ data, int64_t n, double* __restrict__ out )
+{
+    __shared__ double red[ 256 ];
+    double s = 0.0;
+    for ( int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256 )
+    {
+        const double v = data[ i ];
+        s += v * v;
+    }
+    red[ threadIdx.x ] = s;
+    __syncthreads();
+    for ( int k = 128; k > 0; k >>= 1 )
+    {
+        if ( threadIdx.x < k )
+            red[ threadIdx.x ] += red[ threadIdx.x + k ];
+        __syncthreads();
+    }
+    if ( threadIdx.x == 0 )
+        out[ blockIdx.x ] = red[ 0 ];
+}
+
 // theta_0 = w / wL for every user (apsm_noma_detector.cu:321-329).
 __global__ void apsm_lls_init_theta( const float* __restrict__ w, float* __restrict__ thOld, int L, float wL )
 {
     int i = threadIdx.x;
     if ( i < L )
         thOld[ blockIdx.x * L + i ] = w[ i ] / wL;
+}
+
+// Gaussian quadratic form c^T K c over G complex dictionary entries (2G real rows), the
+// Gaussian norm of computeNormRatio (apsm_noma_detector.cu:220-319). Thread i owns complex
+// entry i; the j entries stream through LDS in tiles of 64. For entries i, j with
+// s = a_i . b_j the four real-row kernel values are K(a_i,a_j) = K(b_i,b_j) = E(n - 2p),
+// K(a_i,b_j) = E(n - 2s) and K(b_i,a_j) = E(n + 2s).
+constexpr int kQuadThreads = 256;
+constexpr int kQuadTile = 64;
+__global__ void __launch_bounds__( kQuadThreads ) apsm_quad_form_partial( const float* __restrict__ Z, const float* __restrict__ Zn,
+                                                                        const float* __restrict__ C, int64_t G, int N, float g2,
+                                                                        double* __restrict__ part )
+{
+    const int L = 2 * N;
+    extern __shared__ float sZ[]; // [kQuadTile][L] then [kQuadTile] norms, [kQuadTile][2] coefficients
+    float* sN = sZ + kQuadTile * L;
+    float* sC = sN + kQuadTile;
+    __shared__ double red[ kQuadThreads ];
+    const int64_t i = (int64_t)blockIdx.x * kQuadThreads + threadIdx.x;
+    const bool ok = i < G;
+    const float* zi = Z + ( ok ? i : 0 ) * L;
+    const float ni = ok ? Zn[ i ] : 0.f;
+    const float cai = ok ? C[ 2 * i ] : 0.f, cbi = ok ? C[ 2 * i + 1 ] : 0.f;
+    float acc = 0.f;
+    for ( int64_t j0 = 0; j0 < G; j0 += kQuadTile )
+    {
+        const int nj = (int)( G - j0 < kQuadTile ? G - j0 : kQuadTile );
+        __syncthreads();
+        for ( int e = threadIdx.x; e < nj * L; e += kQuadThreads )
+            sZ[ e ] = Z[ j0 * L + e ];
+        for ( int e = threadIdx.x; e < nj; e += kQuadThreads )
+        {
+            sN[ e ] = Zn[ j0 + e ];
+            sC[ 2 * e ] = C[ 2 * ( j0 + e ) ];
+            sC[ 2 * e + 1 ] = C[ 2 * ( j0 + e ) + 1 ];
+        }
+        __syncthreads();
+        if ( ok )
+            for ( int j = 0; j < nj; j++ )
+            {
+                const float* zj = sZ + j * L;
+                float p = 0.f, s = 0.f;
+                for ( int n = 0; n < N; n++ )
+                {
+                    const float xr = zi[ n ], xi = zi[ N + n ], zr = zj[ n ], zim = zj[ N + n ];
+                    p += xr * zr + xi * zim;
+                    s += xr * zim - xi * zr;
+                }
+                const float nn = ni + sN[ j ];
+                const float E0 = gaussFromDist( nn - 2.f * p, g2 );
+                const float Eab = gaussFromDist( nn - 2.f * s, g2 );
+                const float Eba = gaussFromDist( nn + 2.f * s, g2 );
+                const float caj = sC[ 2 * j ], cbj = sC[ 2 * j + 1 ];
+                acc += ( cai * caj + cbi * cbj ) * E0 + cai * cbj * Eab + cbi * caj * Eba;
+            }
+    }
+    red[ threadIdx.x ] = acc;
+    __syncthreads();
+    for ( int k = kQuadThreads / 2; k > 0; k >>= 1 )
+    {
+        if ( threadIdx.x < k )
+            red[ threadIdx.x ] += red[ threadIdx.x + k ];
+        __syncthreads();
+    }
+    if ( threadIdx.x == 0 )
+        part[ blockIdx.x ] = red[ 0 ];
+}
+
+// Per-user EVM / SER / BER of soft estimates against the true symbols, with the rules of
+// cpp/lib/noma/noma_train_history.cu:105-236 (decision regions for SER, Gray-demapped PAM
+// levels for BER). One workgroup per user: y_true [U][n][2] (row stride ldT complex),
+// y_hat [U][n][2] (row stride ldH complex); out[u*outStride + 0..2] = evm, ser, ber.
+constexpr int kMetricThreads = 1024;
+__device__ __forceinline__ bool componentSymbolError( float t, float h, float scale, float maxVal )
+{
+    return fabsf( t - h ) > scale && !( h > maxVal && t > maxVal - scale ) && !( h < -maxVal && t < -maxVal + scale );
+}
+__device__ __forceinline__ unsigned grayLevel( float x, int levels )
+{
+    int xi = (int)roundf( ( x + (float)( levels - 1 ) ) * 0.5f );
+    xi = min( levels - 1, max( 0, xi ) );
+    return (unsigned)( xi ^ ( xi >> 1 ) );
+}
+__global__ void __launch_bounds__( kMetricThreads ) apsm_history_metrics( const float* __restrict__ yTrue, int64_t ldT,
+                                                                        const float* __restrict__ yHat, int64_t ldH, int64_t n,
+                                                                        int bps, float scale, float* __restrict__ out,
+                                                                        int outStride )
+{
+    const int u = blockIdx.x;
+    const float2* t2 = reinterpret_cast<const float2*>( yTrue ) + (int64_t)u * ldT;
+    const float2* h2 = reinterpret_cast<const float2*>( yHat ) + (int64_t)u * ldH;
+    const int levels = bps == 1 ? 2 : 1 << ( bps / 2 );
+    const float maxVal = (float)( ( 1 << ( bps / 2 ) ) - 1 ) * scale;
+    const float rev = 1.f / scale;
+    double se = 0.0;
+    unsigned sym = 0, bits = 0;
+    for ( int64_t i = threadIdx.x; i < n; i += kMetricThreads )
+    {
+        const float2 t = t2[ i ], h = h2[ i ];
+        const float dr = t.x - h.x, di = t.y - h.y;
+        se += (double)( dr * dr + di * di );
+        if ( bps == 1 )
+            sym += signbit( t.x ) != signbit( h.x );
+        else
+            sym += componentSymbolError( t.x, h.x, scale, maxVal ) || componentSymbolError( t.y, h.y, scale, maxVal );
+        bits += __popc( grayLevel( t.x * rev, levels ) ^ grayLevel( h.x * rev, levels ) );
+        if ( bps > 1 )
+            bits += __popc( grayLevel( t.y * rev, levels ) ^ grayLevel( h.y * rev, levels ) );
+    }
+    __shared__ double rs[ kMetricThreads ];
+    __shared__ unsigned rsym[ kMetricThreads ], rbit[ kMetricThreads ];
+    rs[ threadIdx.x ] = se;
+    rsym[ threadIdx.x ] = sym;
+    rbit[ threadIdx.x ] = bits;
+    __syncthreads();
+    for ( int k = kMetricThreads / 2; k > 0; k >>= 1 )
+    {
+        if ( threadIdx.x < k )
+        {
+            rs[ threadIdx.x ] += rs[ threadIdx.x + k ];
+            rsym[ threadIdx.x ] += rsym[ threadIdx.x + k ];
+            rbit[ threadIdx.x ] += rbit[ threadIdx.x + k ];
+        }
+        __syncthreads();
+    }
+    if ( threadIdx.x == 0 )
+    {
+        out[ (int64_t)u * outStride + 0 ] = (float)sqrt( rs[ 0 ] / (double)n );
+        out[ (int64_t)u * outStride + 1 ] = (float)( (double)rsym[ 0 ] / (double)n );
+        out[ (int64_t)u * outStride + 2 ] = (float)( (double)rbit[ 0 ] / ( (double)n * bps ) );
+    }
 }
 
 __global__ void fill_zero( float* __restrict__ p, int64_t n )

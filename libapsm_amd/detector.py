@@ -62,7 +62,7 @@ class apsm_parameters:  # noqa: N801 - reference class name
             float(self.gaussianKernelWeight), float(self.gaussianKernelVariance), float(self.eB),
             float(self.normConstraint), int(self.windowSize), int(self.sampleStep), int(self.trainPasses),
             int(self.dictionarySize), int(bool(self.startWithFullWindow)), int(bool(self.llsInitialization)),
-            int(bool(self.trainingShuffle)), 0, int(self.detectVersion))
+            int(bool(self.trainingShuffle)), 0, int(self.detectVersion), float(self.linearKernelWeight))
 
 
 def _rx_array(rx, num_antennas=None):
@@ -81,6 +81,33 @@ def _tx_array(tx, num_users, T):
     if a.shape != (num_users, T):
         raise ValueError(f"pilots must be [{num_users}][{T}], got {list(a.shape)}")
     return a
+
+
+class NomaTrainHistory:
+    """Per-step EVM / SER / BER on the training and validation sets
+    (cpp/lib/noma/noma_train_history.cuh:64-99)."""
+
+    def __init__(self, bitPerSymbol, modulationScale):
+        self.bitPerSymbol = int(bitPerSymbol)
+        self.modulationScale = float(modulationScale)
+        self.evmTrain, self.serTrain, self.berTrain = [], [], []
+        self.evmVal, self.serVal, self.berVal = [], [], []
+
+    def extend(self, rows):
+        for r in np.asarray(rows, np.float32):
+            self.evmTrain.append(float(r[0]))
+            self.serTrain.append(float(r[1]))
+            self.berTrain.append(float(r[2]))
+            self.evmVal.append(float(r[3]))
+            self.serVal.append(float(r[4]))
+            self.berVal.append(float(r[5]))
+
+    def toCsv(self, filename):
+        with open(filename, "w") as f:
+            f.write("step,train_evm,train_ser,train_ber,val_evm,val_ser,val_ber\n")
+            for i in range(len(self.evmTrain)):
+                f.write(f"{i},{self.evmTrain[i]:g},{self.serTrain[i]:g},{self.berTrain[i]:g},"
+                        f"{self.evmVal[i]:g},{self.serVal[i]:g},{self.berVal[i]:g}\n")
 
 
 class NomaDetector:
@@ -120,14 +147,49 @@ class ApsmNomaDetector(NomaDetector):
         return self._h
 
     def train(self, rxSigTraining, txSigTraining, valSet=None):
-        if valSet is not None:
-            raise ValueError("training history (valSet) is not supported by this detector")
+        """valSet = (NomaTrainHistory, rx_val, tx_val) records per-step metrics
+        (cpp/lib/apsm/apsm_train.cu:147-158); with several users the history holds user 0."""
         rx = _rx_array(rxSigTraining, self.numAntennas)
         tx = _tx_array(txSigTraining, self.numUsers, rx.shape[1])
         ms = ctypes.c_float()
-        check(lib().apsm_train(self._h, rx.ctypes.data, tx.ctypes.data, rx.shape[1], ctypes.byref(ms)))
+        if valSet is None:
+            check(lib().apsm_train(self._h, rx.ctypes.data, tx.ctypes.data, rx.shape[1], ctypes.byref(ms)))
+        else:
+            hist, rxv, txv = valSet
+            rxv = _rx_array(rxv, self.numAntennas)
+            txv = _tx_array(txv, self.numUsers, rxv.shape[1])
+            per_user = self.train_history(rx, tx, rxv, txv, hist.bitPerSymbol, hist.modulationScale, ms)
+            hist.extend(per_user[:, 0, :])
         self.last_train_ms = ms.value
         return ms.value
+
+    def train_history(self, rx, tx, rxv, txv, bits_per_symbol, modulation_scale, ms=None):
+        """Per-step metrics [steps][users][6] = train EVM, SER, BER, validation EVM, SER, BER."""
+        rx = _rx_array(rx, self.numAntennas)
+        tx = _tx_array(tx, self.numUsers, rx.shape[1])
+        rxv = _rx_array(rxv, self.numAntennas)
+        txv = _tx_array(txv, self.numUsers, rxv.shape[1])
+        cap = rx.shape[1] + 1
+        out = np.zeros((cap, self.numUsers, 6), np.float32)
+        n = ctypes.c_uint32()
+        ms = ms if ms is not None else ctypes.c_float()
+        check(lib().apsm_train_history(self._h, rx.ctypes.data, tx.ctypes.data, rx.shape[1], rxv.ctypes.data,
+                                       txv.ctypes.data, rxv.shape[1], int(bits_per_symbol), float(modulation_scale),
+                                       out.ctypes.data, cap, ctypes.byref(n), ctypes.byref(ms)))
+        return out[:n.value]
+
+    def computeNormRatio(self, user=0):
+        r = ctypes.c_float()
+        check(lib().apsm_norm_ratio(self._h, int(user), ctypes.byref(r)))
+        return r.value
+
+    def getDictionary(self):
+        """Complex basis vectors of the live dictionary, [N][entries], oldest first."""
+        n = ctypes.c_uint32()
+        check(lib().apsm_get_dictionary(self._h, None, 0, ctypes.byref(n)))
+        d = np.zeros((self.numAntennas, max(1, n.value)), np.complex64)
+        check(lib().apsm_get_dictionary(self._h, d.ctypes.data, max(1, n.value), ctypes.byref(n)))
+        return d[:, :n.value]
 
     def detect(self, rxSigData):
         rx = _rx_array(rxSigData, self.numAntennas)
@@ -249,9 +311,31 @@ class LlsNomaDetector(NomaDetector):
         check(lib().lls_synchronize(self._h))
 
     def getWeightMat(self):
-        w = np.zeros(2 * self.numAntennas, np.float32)
+        w = np.zeros(128, np.float32)
         check(lib().lls_get_weights(self._h, w.ctypes.data))
-        return w
+        return w[:self._nW]
+
+    _nW_override = None
+
+    @property
+    def _nW(self):
+        return self._nW_override or 2 * self.numAntennas
+
+    def trainLowLevel(self, rxSigMat, txSigMat, numInputDims, numOutputDims, numSamples):
+        """Device pointers: column-major A (numSamples x numInputDims) and y (numSamples)."""
+        check(lib().lls_train_lowlevel(self._h, ctypes.c_void_p(rxSigMat), ctypes.c_void_p(txSigMat),
+                                       int(numInputDims), int(numOutputDims), int(numSamples)))
+        self._nW_override = int(numInputDims)
+
+    def detectLowLevel(self, rxSigMat, estMat, numInputDims, numOutputDims, numSamples, alpha=1.0, beta=0.0):
+        check(lib().lls_detect_lowlevel(self._h, ctypes.c_void_p(rxSigMat), ctypes.c_void_p(estMat), int(numInputDims),
+                                        int(numOutputDims), int(numSamples), float(alpha), float(beta)))
+
+    def computeSignalPower(self, data, numValues):
+        """sqrt(sum d^2 / n) of a device array (lls_detector.cu:236-242)."""
+        out = ctypes.c_float()
+        check(lib().lls_signal_power(self._h, ctypes.c_void_p(data), int(numValues), ctypes.byref(out)))
+        return out.value
 
     def resetState(self):
         pass  # state is replaced by the next train (lls_detector.cu:220-223)

@@ -20,6 +20,8 @@
 //                      and :333-428 (window projection, soft threshold, normalisation,
 //                      coefficient and linear-weight update from the pre-step state).
 //   norm constraint    cpp/lib/apsm/apsm_train.cu:192-256.
+//   norm ratio         cpp/lib/apsm/apsm_noma_detector.cu:220-319.
+//   train history      cpp/lib/noma/noma_train_history.cu:105-236 (per-step EVM/SER/BER).
 //   variance heuristic cpp/lib/apsm/apsm_noma_detector.cu:194-218.
 //   shuffle            cpp/lib/noma/noma_detector.cu:59-74.
 //   LLS                cpp/lib/noma/lls_detector.cu:136-218 (least squares over the real
@@ -66,7 +68,7 @@ namespace {
 
 struct OrcParams
 {
-    double gaussian_kernel_weight; // wG; wL = 1 - wG
+    double gaussian_kernel_weight; // wG
     double gaussian_kernel_variance; // JSON value v; 0 selects the heuristic
     double eB;
     double norm_constraint;
@@ -78,6 +80,7 @@ struct OrcParams
     int32_t training_shuffle;
     int32_t lls_initialization;
     int32_t pad;
+    double linear_kernel_weight; // wL (the reference factory sets 1 - wG)
 };
 
 thread_local std::string g_err;
@@ -580,7 +583,7 @@ Model<R>* createImpl( const OrcParams* p, int N )
     m->L = 2 * N;
     m->H = 2 * (int64_t)p->dictionary_size;
     m->wG = R( p->gaussian_kernel_weight );
-    m->wL = R( 1 ) - R( p->gaussian_kernel_weight );
+    m->wL = R( p->linear_kernel_weight );
     m->theta.assign( m->L, 0 );
     return m;
 }
@@ -677,6 +680,20 @@ int64_t orc_apsm_state( void* h, double* theta, double* coeffs, int64_t cap, int
     for ( int64_t i = 0; i < n; i++ )
         coeffs[ i ] = m->c[ l + i ];
     return m->used;
+}
+
+// computeNormRatio (apsm_noma_detector.cu:220-319): (wL * |theta|^2) / (wG * sum_ij c_i c_j
+// K(d_i, d_j)) over the dictionary rows [lo, used).
+double orc_apsm_norm_ratio( void* h )
+{
+    const Model<double>* m = static_cast<Model<double>*>( h );
+    const int64_t lo = m->dictLo( m->used );
+    double g = 0;
+    for ( int64_t i = lo; i < m->used; i++ )
+        for ( int64_t j = lo; j < m->used; j++ )
+            g += gauss( &m->X[ i * m->L ], &m->X[ j * m->L ], m->L, m->var ) * m->c[ i ] * m->c[ j ];
+    const double lin = dot( m->theta.data(), m->theta.data(), m->L );
+    return ( m->wL * lin ) / ( m->wG * g );
 }
 
 // f(u) for arbitrary real rows u [n][L] under the current state (for kernel-level checks).

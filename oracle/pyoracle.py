@@ -31,6 +31,7 @@ class OrcParams(ctypes.Structure):
         ("training_shuffle", ctypes.c_int32),
         ("lls_initialization", ctypes.c_int32),
         ("pad", ctypes.c_int32),
+        ("linear_kernel_weight", ctypes.c_double),
     ]
 
 
@@ -53,11 +54,12 @@ DEFAULTS = dict(
 def make_params(cfg):
     c = dict(DEFAULTS)
     c.update({k: v for k, v in cfg.items() if k not in ("otype", "detectVersion")})
+    wL = float(c.get("linearKernelWeight", 1.0 - float(c["gaussianKernelWeight"])))
     return OrcParams(
         float(c["gaussianKernelWeight"]), float(c["gaussianKernelVariance"]), float(c["eB"]),
         float(c["normConstraint"]), int(c["windowSize"]), int(c["sampleStep"]),
         int(c["trainPasses"]), int(c["dictionarySize"]), int(bool(c["startWithFullWindow"])),
-        int(bool(c["trainingShuffle"])), int(bool(c["llsInitialization"])), 0)
+        int(bool(c["trainingShuffle"])), int(bool(c["llsInitialization"])), 0, wL)
 
 
 def build():
@@ -88,6 +90,8 @@ def lib():
         L.orc_apsm_state.restype = i64
         L.orc_apsm_state.argtypes = [ctypes.c_void_p, f64p, f64p, i64, P(i64), f64p]
         L.orc_apsm_eval_rows.argtypes = [ctypes.c_void_p, f64p, i64, f64p]
+        L.orc_apsm_norm_ratio.restype = ctypes.c_double
+        L.orc_apsm_norm_ratio.argtypes = [ctypes.c_void_p]
         L.orc_apsm_set_state.argtypes = [ctypes.c_void_p, f64p, f64p, f64p, i64, ctypes.c_double]
         L.orc_lls_train.argtypes = [f32p, f32p, ctypes.c_int, i64, ctypes.c_int, f64p]
         L.orc_lls_detect.argtypes = [f32p, ctypes.c_int, i64, f64p, f64p]
@@ -192,6 +196,9 @@ class Apsm:
         rc = self.L.orc_apsm_set_state(self.h, prows, pc, pt, rows.shape[0], float(var))
         if rc != 0:
             raise ValueError(self.L.orc_last_error().decode())
+
+    def norm_ratio(self):
+        return self.L.orc_apsm_norm_ratio(self.h)
 
     def eval_rows(self, u):
         u, pu = _f64(u)
