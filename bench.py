@@ -112,35 +112,31 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    import torch
-
-    from libapsm_amd import ApsmNomaDetectorFactory, LlsNomaDetector, datagen
+    from libapsm_amd import ApsmNomaDetectorFactory, LlsNomaDetector, datagen, lib
+    from libapsm_amd._lib import DeviceArray, check
 
     w = datagen.WORKLOADS[args.workload]
     N, U, T, M = w["N"], len(w["users"]), w["T"], w["M"]
     data = datagen.generate(N, w["U_tx"], T, M, modulation=w["modulation"], snr_db=w["snr_db"], seed=w["seed"],
                             large_scale_db=w["large_scale_db"])
     tx_train = np.ascontiguousarray(data["tx_train"][w["users"]])
-    dev = torch.device("cuda:0")
-
     def to_dev(z):
-        return torch.from_numpy(np.ascontiguousarray(z).view(np.float32)).to(dev)
+        return DeviceArray(np.ascontiguousarray(z, dtype=np.complex64).view(np.float32))
 
     d_rxt, d_txt, d_rxd = to_dev(data["rx_train"]), to_dev(tx_train), to_dev(data["rx_data"])
-    d_est = torch.empty((U, M, 2), dtype=torch.float32, device=dev)
+    d_est = DeviceArray.empty((U, M, 2), np.float32)
     cfg = datagen.workload_config(w)
     det = ApsmNomaDetectorFactory.build(cfg, N, U)
-    torch.cuda.synchronize()
 
     def step():
         det.resetState()
-        det.train_device(d_rxt.data_ptr(), d_txt.data_ptr(), T)
-        det.detect_device(d_rxd.data_ptr(), M, d_est.data_ptr())
+        det.train_device(d_rxt.ptr, d_txt.ptr, T)
+        det.detect_device(d_rxd.ptr, M, d_est.ptr)
 
     for _ in range(args.warmup):
         step()
     det.synchronize()
-    torch.cuda.synchronize()
+    check(lib().apsm_device_synchronize())
     t_train_k, t_det_k = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -149,12 +145,12 @@ def main():
         t_train_k.append(a)
         t_det_k.append(b)
     det.synchronize()
-    torch.cuda.synchronize()
+    check(lib().apsm_device_synchronize())
     elapsed = time.perf_counter() - t0
     ms_step = 1000.0 * elapsed / args.steps
     value = M * U / (ms_step / 1000.0)
 
-    est = d_est.cpu().numpy()
+    est = d_est.numpy()
     est_c = est[..., 0] + 1j * est[..., 1]
     tx_data = data["tx_data"][w["users"]]
     ser = datagen.symbol_errors(est_c, tx_data, data["bps"]) / est_c.size
@@ -199,14 +195,13 @@ def main():
     if not args.no_lls:
         lls = LlsNomaDetector(N)
         d_txu = to_dev(data["tx_train"][w["users"][0]])
-        d_el = torch.empty((M, 2), dtype=torch.float32, device=dev)
-        torch.cuda.synchronize()
+        d_el = DeviceArray.empty((M, 2), np.float32)
         t0 = time.perf_counter()
-        lls.train_device(d_rxt.data_ptr(), d_txu.data_ptr(), T)
-        lls.detect_device(d_rxd.data_ptr(), M, d_el.data_ptr())
+        lls.train_device(d_rxt.ptr, d_txu.ptr, T)
+        lls.detect_device(d_rxd.ptr, M, d_el.ptr)
         lls.synchronize()
         t_lls = time.perf_counter() - t0
-        el = d_el.cpu().numpy()
+        el = d_el.numpy()
         u0 = w["users"][0]
         out["lls"] = {"user": u0, "ms": 1000 * t_lls,
                       "ser": datagen.symbol_errors(el[:, 0] + 1j * el[:, 1], data["tx_data"][u0], data["bps"]) / M,

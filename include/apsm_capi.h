@@ -189,6 +189,16 @@ int apsm_stream_synchronize(void* stream);
 /* Number of visible HIP devices (0 when no GPU is present). Never fails. */
 int apsm_device_count(void);
 
+/* Device memory plumbing for callers without a device runtime of their own (tests, the
+ * Python mirror, the benchmark); the reference allocates inside its matrix classes
+ * (cpp/lib/cuda/cuda_matrix_host.cuh). Copies are synchronous with the whole device: a
+ * download waits for the work of every stream, an upload completes before it returns. */
+int apsm_device_malloc(void** ptr, uint64_t bytes);
+int apsm_device_free(void* ptr);
+int apsm_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
+int apsm_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
+int apsm_device_synchronize(void);
+
 #ifdef __cplusplus
 }
 #endif

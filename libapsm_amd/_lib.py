@@ -76,6 +76,11 @@ SIGNATURES = {
     "lls_signal_power": (ctypes.c_int, [_vp, _vp, _u32, _f32p]),
     "lls_weights_dev": (_vp, [_vp]),
     "lls_stream": (_vp, [_vp]),
+    "apsm_device_malloc": (ctypes.c_int, [_P(_vp), ctypes.c_uint64]),
+    "apsm_device_free": (ctypes.c_int, [_vp]),
+    "apsm_memcpy_h2d": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
+    "apsm_memcpy_d2h": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
+    "apsm_device_synchronize": (ctypes.c_int, []),
     "apsm_stream_create": (ctypes.c_int, [_P(_vp)]),
     "apsm_stream_destroy": (ctypes.c_int, [_vp]),
     "apsm_stream_synchronize": (ctypes.c_int, [_vp]),
@@ -104,3 +109,48 @@ def lib():
 def check(rc):
     if rc != APSM_OK:
         raise ApsmError(rc, lib().apsm_last_error().decode())
+
+
+class DeviceArray:
+    """A device allocation of the library's HIP runtime holding one numpy array's bytes.
+
+    DeviceArray(a) uploads a; DeviceArray.empty(shape, dtype) allocates; .ptr is the device
+    address; .numpy() downloads (synchronous)."""
+
+    def __init__(self, a=None, shape=None, dtype=None):
+        import numpy as np
+        if a is not None:
+            a = np.ascontiguousarray(a)
+            shape, dtype = a.shape, a.dtype
+        self.shape, self.dtype = tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self._p = ctypes.c_void_p()
+        check(lib().apsm_device_malloc(ctypes.byref(self._p), self.nbytes))
+        if a is not None:
+            check(lib().apsm_memcpy_h2d(self._p, a.ctypes.data, self.nbytes))
+
+    @classmethod
+    def empty(cls, shape, dtype):
+        return cls(shape=shape, dtype=dtype)
+
+    @property
+    def ptr(self):
+        return self._p.value or 0
+
+    def numpy(self):
+        import numpy as np
+        out = np.empty(self.shape, self.dtype)
+        check(lib().apsm_memcpy_d2h(out.ctypes.data, self._p, self.nbytes))
+        return out
+
+    def upload(self, a):
+        import numpy as np
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(lib().apsm_memcpy_h2d(self._p, a.ctypes.data, self.nbytes))
+
+    def __del__(self):
+        p = getattr(self, "_p", None)
+        if p is not None and p.value and _lib is not None:
+            _lib.apsm_device_free(p)
+            self._p = ctypes.c_void_p()

@@ -115,13 +115,15 @@ int run( int argc, char** argv )
     const nlohmann::json algorithm = config[ "algorithm" ];
     if ( args.flag( "--parallel-processing" ) )
     {
-        const uint32_t U = (uint32_t)txTrain.size();
+        // the reference tool detects its default user count (cpp/cli/common/util.hpp:103); the
+        // bundled files carry two more, all-zero user columns
+        const uint32_t U = std::min<uint32_t>( cli::kDefaultUsers, (uint32_t)txTrain.size() );
         ComplexSampleMatrix est( U );
         if ( algorithm[ "otype" ] == "APSM" )
         {
             auto det = ApsmNomaDetectorFactory::build( algorithm, numAntennas, U );
             auto* apsm = dynamic_cast<ApsmDetectorBase*>( det.get() );
-            apsm->train( rxTrain, txTrain );
+            apsm->train( rxTrain, ComplexSampleMatrix( txTrain.begin(), txTrain.begin() + U ) );
             apsm->detect( rxData, est );
         }
         else

@@ -781,6 +781,53 @@ int apsm_device_count( void )
     return n;
 }
 
+int apsm_device_malloc( void** ptr, uint64_t bytes )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( ptr != nullptr, "null output pointer" );
+        *ptr = nullptr;
+        if ( bytes )
+            HIP_OR_THROW( hipMalloc( ptr, bytes ) );
+    } );
+}
+
+int apsm_device_free( void* ptr )
+{
+    return guarded( [ & ] {
+        if ( ptr )
+            HIP_OR_THROW( hipFree( ptr ) );
+    } );
+}
+
+int apsm_memcpy_h2d( void* dst, const void* src, uint64_t bytes )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( ( dst && src ) || bytes == 0, "null argument" );
+        if ( bytes )
+        {
+            HIP_OR_THROW( hipMemcpy( dst, src, bytes, hipMemcpyHostToDevice ) );
+            HIP_OR_THROW( hipDeviceSynchronize() );
+        }
+    } );
+}
+
+int apsm_memcpy_d2h( void* dst, const void* src, uint64_t bytes )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( ( dst && src ) || bytes == 0, "null argument" );
+        if ( bytes )
+        {
+            HIP_OR_THROW( hipDeviceSynchronize() ); // work on every stream, non-blocking ones included
+            HIP_OR_THROW( hipMemcpy( dst, src, bytes, hipMemcpyDeviceToHost ) );
+        }
+    } );
+}
+
+int apsm_device_synchronize( void )
+{
+    return guarded( [ & ] { HIP_OR_THROW( hipDeviceSynchronize() ); } );
+}
+
 int apsm_create( const apsm_params* p, uint32_t num_antennas, uint32_t num_users, apsm_handle** out )
 {
     return guarded( [ & ] {
