@@ -105,6 +105,10 @@ def kernel_variance_for_sigma(sigma, num_antennas):
 
 # Workloads (sizes from the benchmark configurations).
 WORKLOADS = {
+    # 64 antennas, 12 QPSK users at 12 dB; dictionary of 16384 centres; 8M detect samples
+    "mixed-64ant-qpsk": dict(N=64, U_tx=12, users=list(range(12)), T=16384, M=8000000, modulation="QPSK",
+                             snr_db=12.0, seed=4, large_scale_db=None, window=80, sigma=0.5, eB=0.01,
+                             dict_size=16384),
     # 6 NOMA users, QPSK, 8 antennas, 10 dB; detect user 1; w=80, sigma=0.5, eps=0.01
     "plain-cpu-8ant": dict(N=8, U_tx=6, users=[1], T=20000, M=80000, modulation="QPSK", snr_db=10.0, seed=1,
                            large_scale_db=None, window=80, sigma=0.5, eB=0.01, dict_size=20000),
@@ -120,3 +124,77 @@ def workload_config(w):
     return {"otype": "APSM", "windowSize": w["window"], "eB": w["eB"],
             "gaussianKernelVariance": kernel_variance_for_sigma(w["sigma"], w["N"]),
             "dictionarySize": w["dict_size"]}
+
+
+def write_reference_files(data, out_dir, name):
+    """Write a generated data set in the reference tool's formats: the synchronized received
+    signals (keys rxSigTraining, rxSigData; cpp/cli/common/argparse_helper.cpp:160-178) and the
+    reference symbols as unscaled integer constellation levels (keys mod_training, mod_data;
+    scaled by the modulation scale when loaded, argparse_helper.cpp:101-102). Returns the two
+    paths (rx file, tx file)."""
+    import os
+
+    from . import binfile
+    os.makedirs(out_dir, exist_ok=True)
+    s = mod_scale(data["bps"])
+    rx_path = os.path.join(out_dir, f"rxData_{name}.bin")
+    tx_path = os.path.join(out_dir, f"NOMA_signals_{name}.bin")
+    binfile.write(rx_path, {"rxSigTraining": binfile.from_complex(data["rx_train"]),
+                            "rxSigData": binfile.from_complex(data["rx_data"])})
+    binfile.write(tx_path, {"mod_training": binfile.from_complex(np.round(data["tx_train"] / s)),
+                            "mod_data": binfile.from_complex(np.round(data["tx_data"] / s))})
+    return rx_path, tx_path
+
+
+def batched_instances(count=512, seed=3):
+    """Parameter draws of the batched workload: per instance antennas in {4, 8, 16, 32},
+    QPSK or 16QAM, dictionary size in [200, 4000], windowSize in {40, 80, 160}, 10k..100k
+    detect samples; pilots = dictionary size; instance i is generated from seed 3e6 + i."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(count):
+        dc = int(rng.integers(200, 4001))
+        out.append(dict(N=int(rng.choice([4, 8, 16, 32])), modulation=str(rng.choice(["QPSK", "QAM16"])), dict_size=dc,
+                        window=int(rng.choice([40, 80, 160])), M=int(rng.integers(10000, 100001)), T=dc,
+                        seed=3000000 + i, U_tx=4, user=0, snr_db=15.0, sigma=0.5, eB=0.01))
+    return out
+
+
+def serving_frames(count, seed=5, links=2048, rate_hz=1000.0):
+    """Requests of the serving workload: Poisson arrivals at rate_hz; each frame carries
+    14 OFDM symbols x (12 x U[1, 273]) subcarriers, 5 % of them pilots, on a link drawn
+    uniformly from `links` (8..32 antennas, QPSK or 16QAM per link)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    link_par = [dict(N=int(rng.integers(8, 33)), modulation=str(rng.choice(["QPSK", "QAM16"]))) for _ in range(links)]
+    t = 0.0
+    frames = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate_hz)
+        n = 14 * 12 * int(rng.integers(1, 274))
+        pilots = max(1, int(round(0.05 * n)))
+        link = int(rng.integers(0, links))
+        frames.append(dict(arrival_s=t, link=link, samples=n, pilots=pilots, seed=5000000 + i, **link_par[link]))
+    return frames
+
+
+def _main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="Write a synthetic NOMA data set in the reference tool's file formats.")
+    ap.add_argument("workload", choices=sorted(WORKLOADS))
+    ap.add_argument("out_dir")
+    ap.add_argument("--pilots", type=int, help="override the number of pilot samples")
+    ap.add_argument("--samples", type=int, help="override the number of detect samples")
+    a = ap.parse_args(argv)
+    w = dict(WORKLOADS[a.workload])
+    T = a.pilots or w["T"]
+    M = a.samples or w["M"]
+    d = generate(w["N"], w["U_tx"], T, M, modulation=w["modulation"], snr_db=w["snr_db"], seed=w["seed"],
+                 large_scale_db=w["large_scale_db"])
+    rx, tx = write_reference_files(d, a.out_dir, a.workload)
+    cfg = {"algorithm": workload_config(w), "modulation": w["modulation"], "user": w["users"][0]}
+    import json
+    print(json.dumps({"rx": rx, "tx": tx, "config": cfg}))
+
+
+if __name__ == "__main__":
+    _main()

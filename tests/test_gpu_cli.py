@@ -85,3 +85,24 @@ def test_tool_random_reference(require_gpu):
                    '{"algorithm": {"otype": "LLS"}, "modulation": "QPSK"}', "-o", "scripting")
     (ber, ser, evm), = scripting_values(out)
     assert 0.0 <= ber <= 100.0 and 0.0 <= ser <= 100.0 and evm > 0.0
+
+
+def test_tool_on_generated_files(require_gpu, oracle, tmp_path):
+    """The generator's files feed the tool directly; the tool's SER equals the API's on the
+    same data set."""
+    import json
+
+    from libapsm_amd import NomaDetectorFactory, datagen
+    d = datagen.generate(8, 6, 1500, 3000, modulation="QPSK", snr_db=10.0, seed=1)
+    rx, tx = datagen.write_reference_files(d, str(tmp_path), "small")
+    cfg = {"otype": "APSM", "windowSize": 40, "eB": 0.01, "gaussianKernelVariance": datagen.kernel_variance_for_sigma(0.5, 8),
+           "dictionarySize": 1500}
+    out = run_tool("-s", rx, "-r", tx, "-cs", json.dumps({"algorithm": cfg, "modulation": "QPSK", "user": 1}), "-o", "scripting")
+    (ber, ser, evm), = scripting_values(out)
+    det = NomaDetectorFactory.build(cfg, 8)
+    det.train(d["rx_train"], d["tx_train"][1])
+    e = to_ri(det.detect(d["rx_data"]))
+    ref = to_ri(d["tx_data"][1])
+    assert ser == pytest.approx(100.0 * oracle.symbol_errors(e, ref, 2, cli_compat=True) / 3000, abs=1e-3)
+    assert evm == pytest.approx(oracle.evm(e, ref), rel=1e-4)
+    assert ser < 5.0
