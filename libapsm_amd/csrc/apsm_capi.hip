@@ -7,6 +7,7 @@
 #include "apsm_capi.h"
 #include "apsm_device.h"
 #include "apsm_train2.h"
+#include "apsm_train_nc.h"
 
 #include <hip/hip_runtime.h>
 
@@ -274,6 +275,8 @@ struct apsm_handle
     bool blocked = false; // blocked train path (register-resident window Gram)
     int64_t a0Next = 0; // blocked path: frozen boundary after the last step (absolute real row)
     DevBuf<float> Pbuf, Gact, EvF; // blocked path work buffers
+    DevBuf<float> GramG, FG; // norm-constraint path: Gaussian window Gram and Gaussian part of f
+    DevBuf<double> Qn; // norm-constraint path: c^T K c per user
     DevBuf<float> rx, tx, est; // staging for the host entry points
     DevBuf<int32_t> perm;
     lls_handle* lls = nullptr;
@@ -289,6 +292,12 @@ void resetDeviceState( apsm_handle* h )
         HIP_OR_THROW( hipMemsetAsync( h->Gram.ptr, 0, nG * sizeof( float ), h->stream ) );
     HIP_OR_THROW( hipMemsetAsync( h->thOld.ptr, 0, nT * sizeof( float ), h->stream ) );
     HIP_OR_THROW( hipMemsetAsync( h->theta.ptr, 0, nT * sizeof( float ), h->stream ) );
+    if ( h->GramG.ptr )
+        HIP_OR_THROW( hipMemsetAsync( h->GramG.ptr, 0, nG * sizeof( float ), h->stream ) );
+    if ( h->FG.ptr )
+        HIP_OR_THROW( hipMemsetAsync( h->FG.ptr, 0, nF * sizeof( float ), h->stream ) );
+    if ( h->Qn.ptr )
+        HIP_OR_THROW( hipMemsetAsync( h->Qn.ptr, 0, h->U * sizeof( double ), h->stream ) );
     h->used = 0;
     h->a0Next = 0;
     h->bufBaseC = 0;
@@ -312,8 +321,9 @@ void validateParams( const apsm_params* p, uint32_t N, uint32_t U )
     ARG_CHECK( !p->lls_initialization || p->linear_kernel_weight > 0.f, "llsInitialization needs a positive linear kernel weight" );
     ARG_CHECK( p->gaussian_kernel_variance >= 0.f, "gaussianKernelVariance must be non-negative" );
     ARG_CHECK( p->eB >= 0.f, "eB must be non-negative" );
-    if ( p->norm_constraint > 0.f )
-        throw ApsmError { APSM_ERR_UNSUPPORTED, "normConstraint > 0 is not implemented on the device path" };
+    ARG_CHECK( p->norm_constraint >= 0.f, "normConstraint must be non-negative" );
+    if ( p->norm_constraint > 0.f && p->sample_step > 2 * apsm::kNewMax )
+        throw ApsmError { APSM_ERR_UNSUPPORTED, "normConstraint > 0 supports sampleStep <= 8" };
     switch ( p->detect_version )
     {
     case APSM_VERSION_DETECT_ORIGINAL:
@@ -330,7 +340,8 @@ void validateParams( const apsm_params* p, uint32_t N, uint32_t U )
 // layouts (windowSize <= 160), at most kNewMax complex samples entering per step.
 bool blockedEligible( const apsm_params& p )
 {
-    return p.train_passes == 1 && !p.start_with_full_window && p.window_size <= 160 && ( p.sample_step == 2 || p.sample_step == 4 );
+    return p.train_passes == 1 && !p.start_with_full_window && p.window_size <= 160 && ( p.sample_step == 2 || p.sample_step == 4 )
+        && p.norm_constraint == 0.f;
 }
 
 constexpr int64_t kBlockRowsMax = 256; // real rows entering per block
@@ -651,6 +662,13 @@ void trainDev( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, 
     HIP_OR_THROW( hipEventRecord( h->ev0, h->stream ) );
     if ( h->blocked )
         trainBlocked( h, nb, ld, (int64_t)h->used, 2 * T, a.rowBase, wL, wG, a.g2 );
+    else if ( p.norm_constraint > 0.f )
+    {
+        apsm::NcArgs nca { h->GramG.ptr, h->FG.ptr, h->Qn.ptr, p.norm_constraint };
+        hipLaunchKernelGGL( apsm::apsm_train_seq_nc, dim3( U ), dim3( apsm::kSeqThreads ), apsm::seqNcLdsBytes( h->w2, L ), h->stream,
+                            nb.Z.ptr, nb.Zn.ptr, nb.Y.ptr, nb.C.ptr, h->F.ptr, h->Gram.ptr, h->thOld.ptr, h->theta.ptr, a, nca );
+        HIP_OR_THROW( hipGetLastError() );
+    }
     else
     {
         hipLaunchKernelGGL( apsm::apsm_train_seq, dim3( U ), dim3( apsm::kSeqThreads ), lds, h->stream, nb.Z.ptr, nb.Zn.ptr, nb.Y.ptr,
@@ -669,6 +687,8 @@ void trainDev( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, 
     h->bufBaseC = usedC - nOld;
     h->bufK = K;
 
+    if ( p.norm_constraint > 0.f )
+        return; // the norm-constraint sequencer keeps theta explicitly and wrote it
     const int64_t newUsedC = (int64_t)( h->used / 2 );
     const int64_t lo = h->blocked ? std::min<int64_t>( h->a0Next / 2, newUsedC ) : std::max<int64_t>( 0, newUsedC - h->Hc );
     hipLaunchKernelGGL( apsm::apsm_theta_full, dim3( U ), dim3( 256 ), 0, h->stream, nb.Z.ptr, nb.C.ptr, h->thOld.ptr,
@@ -853,6 +873,12 @@ int apsm_create( const apsm_params* p, uint32_t num_antennas, uint32_t num_users
             h->F.ensure( (size_t)h->U * h->w2 );
             if ( !h->blocked )
                 h->Gram.ensure( (size_t)h->U * h->w2 * h->w2 );
+            if ( p->norm_constraint > 0.f )
+            {
+                h->GramG.ensure( (size_t)h->U * h->w2 * h->w2 );
+                h->FG.ensure( (size_t)h->U * h->w2 );
+                h->Qn.ensure( (size_t)h->U );
+            }
             h->thOld.ensure( (size_t)h->U * h->L );
             h->theta.ensure( (size_t)h->U * h->L );
             resetDeviceState( h );
@@ -877,6 +903,9 @@ void apsm_destroy( apsm_handle* h )
         b.release();
     h->F.release();
     h->Gram.release();
+    h->GramG.release();
+    h->FG.release();
+    h->Qn.release();
     h->Pbuf.release();
     h->Gact.release();
     h->EvF.release();

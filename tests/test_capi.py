@@ -44,7 +44,7 @@ def test_params_default_matches_reference_factory():
     ("sample_step", 0, -1),
     ("window_size", 700, -1),
     ("detect_version", 12345, -1),
-    ("norm_constraint", 1.5, -4),
+    ("norm_constraint", -1.0, -1),
     ("gaussian_kernel_weight", 1.5, -1),
 ])
 def test_create_rejects_bad_parameters(field, value, code):

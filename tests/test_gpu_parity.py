@@ -78,6 +78,13 @@ CASES = [
     ("w128", dict(windowSize=128, dictionarySize=700, eB=0.01, gaussianKernelVariance=0.25 / 256), 16, 600),
     ("w160-evict", dict(windowSize=160, dictionarySize=400, eB=0.01, gaussianKernelVariance=0.25 / 256), 16, 1500),
     ("w40-q4", dict(windowSize=40, dictionarySize=90, eB=0.001, gaussianKernelVariance=0.25 / 64, sampleStep=4), 8, 500),
+    ("nc", dict(windowSize=8, dictionarySize=150, eB=0.01, gaussianKernelVariance=0.1, normConstraint=0.05), 4, 150),
+    ("nc-fifo-q4", dict(windowSize=8, dictionarySize=30, eB=0.01, gaussianKernelVariance=0.1, normConstraint=0.02,
+                        sampleStep=4), 4, 200),
+    ("nc-passes2", dict(windowSize=6, dictionarySize=150, eB=0.001, gaussianKernelVariance=0.1, normConstraint=0.05,
+                        trainPasses=2), 4, 150),
+    ("nc-n16-w80", dict(windowSize=80, dictionarySize=200, eB=0.01, gaussianKernelVariance=0.25 / 256,
+                        normConstraint=0.05), 16, 400),
 ]
 
 
@@ -174,3 +181,17 @@ def test_bundled_reference_data(require_gpu, oracle, mod, domain):
     e_o = oracle.symbol_errors(est_o, ref, d["bps"])
     print(f"{mod}/{domain}: SER gpu {e_g} oracle {e_o} of {ref.shape[0]}")
     assert abs(e_g - e_o) <= max(1, 0.01 * e_o)
+
+
+def test_norm_constraint_two_calls_and_ratio(require_gpu, oracle):
+    """The norm-constraint state (explicit theta, Gaussian part of f, c^T K c) carries over to
+    the next train call; the device's norm ratio matches the oracle's after the constraint acted."""
+    from libapsm_amd import ApsmNomaDetectorFactory
+    cfg = dict(windowSize=8, dictionarySize=50, eB=0.01, gaussianKernelVariance=0.1, normConstraint=0.03)
+    d = small_problem(N=4, T=160, M=200, seed=12)
+    det = ApsmNomaDetectorFactory.build(dict(cfg, otype="APSM"), 4)
+    det.train(d["rx_train"][:, :90], d["tx_train"][0, :90])
+    det.train(d["rx_train"][:, 90:], d["tx_train"][0, 90:])
+    o, est_o = orc_run(oracle, cfg, 4, d["rx_train"], d["tx_train"][0], d["rx_data"])
+    compare(det.detect(d["rx_data"]), est_o, 2)
+    assert det.computeNormRatio() == pytest.approx(o.norm_ratio(), rel=2e-3)

@@ -28,6 +28,9 @@ CFGS = [
     dict(windowSize=6, dictionarySize=150, eB=0.01, gaussianKernelVariance=0.2, trainingShuffle=True),
     dict(windowSize=6, dictionarySize=150, eB=0.01, gaussianKernelVariance=0.2, llsInitialization=True,
          gaussianKernelWeight=0.3),
+    dict(windowSize=8, dictionarySize=150, eB=0.01, gaussianKernelVariance=0.1, normConstraint=0.3),
+    dict(windowSize=8, dictionarySize=30, eB=0.01, gaussianKernelVariance=0.1, normConstraint=0.3, sampleStep=4),
+    dict(windowSize=6, dictionarySize=150, eB=0.001, gaussianKernelVariance=0.1, normConstraint=0.2, trainPasses=2),
 ]
 
 
@@ -201,3 +204,83 @@ def test_symbol_errors_are_decision_regions(oracle, bps):
     e_orc = oracle.symbol_errors(to_ri(est[ok]), to_ri(true[ok]), bps)
     assert e_orc == e_brute
     assert oracle.bit_errors(to_ri(true), to_ri(true), bps) == 0
+
+
+def _gauss_quad(rows, c, var):
+    """c^T K c for real rows [n][L], K = exp(-0.5/var |r_i - r_j|^2) (numpy, FP64)."""
+    d = ((rows[:, None, :] - rows[None, :, :]) ** 2).sum(-1)
+    return float(c @ np.exp(-0.5 / var * d) @ c)
+
+
+def _real_rows(rx):
+    """Real embedding rows of complex samples [N][S]: row 2k = [Re; Im], row 2k+1 = [Im; -Re]."""
+    x = np.asarray(rx, np.complex128).T
+    a = np.concatenate([x.real, x.imag], 1)
+    b = np.concatenate([x.imag, -x.real], 1)
+    return np.stack([a, b], 1).reshape(-1, a.shape[1])
+
+
+@pytest.mark.parametrize("dc", [300, 40])
+def test_norm_constraint_is_enforced(oracle, dc):
+    """After every step sqrt(wG c^T K c) <= normConstraint (apsm_train.cu:192-256), with a bound
+    at 30 % of the unconstrained final norm so that it binds."""
+    d = small_problem(N=4, T=120, seed=4)
+    base = dict(windowSize=8, dictionarySize=dc, eB=0.01, gaussianKernelVariance=0.1, gaussianKernelWeight=0.5)
+    rows_all = _real_rows(d["rx_train"])
+    free = oracle.Apsm(base, 4)
+    free.train(d["rx_train"], d["tx_train"][0], incremental=True)
+    sf = free.state()
+    rows = rows_all[sf["lo"]:sf["used"]]
+    nc = 0.3 * np.sqrt(0.5 * _gauss_quad(rows, sf["coeffs"], sf["var"]))
+    o = oracle.Apsm(dict(base, normConstraint=nc), 4)
+    o.train(d["rx_train"], d["tx_train"][0], incremental=True)
+    st = o.state()
+    n = np.sqrt(0.5 * _gauss_quad(rows, st["coeffs"], st["var"]))
+    assert n <= nc * (1 + 1e-9)
+    assert not np.allclose(st["theta"], sf["theta"])
+
+
+def test_norm_constraint_inactive_is_neutral(oracle):
+    d = small_problem(N=4, T=100, seed=6)
+    base = dict(windowSize=6, dictionarySize=60, eB=0.01, gaussianKernelVariance=0.1)
+    a, b = oracle.Apsm(base, 4), oracle.Apsm(dict(base, normConstraint=1e9), 4)
+    a.train(d["rx_train"], d["tx_train"][0], incremental=True)
+    b.train(d["rx_train"], d["tx_train"][0], incremental=True)
+    np.testing.assert_allclose(a.detect(d["rx_data"]), b.detect(d["rx_data"]), rtol=0, atol=1e-12)
+
+
+def test_norm_constraint_first_step_closed_form(oracle):
+    """From the zero state the first step gives beta_t = soft(y_t, eB) / 2 / (wL|x|^2 + wG) for
+    the two rows of sample 0; with the bound binding, c = beta * nc / sqrt(wG beta^T K beta)
+    and theta keeps the unscaled beta (only the Gaussian coefficients are scaled)."""
+    d = small_problem(N=3, T=1, seed=2)
+    eB, v, wG, nc = 0.01, 0.1, 0.4, 0.05
+    cfg = dict(windowSize=1, dictionarySize=4, eB=eB, gaussianKernelVariance=v, gaussianKernelWeight=wG, normConstraint=nc)
+    o = oracle.Apsm(cfg, 3)
+    o.train(d["rx_train"], d["tx_train"][0], incremental=False)
+    st = o.state()
+    rows = _real_rows(d["rx_train"])
+    y = np.array([d["tx_train"][0][0].real, d["tx_train"][0][0].imag], np.float64)
+    xn = (np.abs(d["rx_train"][:, 0].astype(np.complex128)) ** 2).sum()
+    beta = np.sign(y) * np.maximum(np.abs(y) - eB, 0) / 2 / ((1 - wG) * xn + wG)
+    var = v * 9
+    n = np.sqrt(wG * _gauss_quad(rows, beta, var))
+    assert n > nc
+    np.testing.assert_allclose(st["coeffs"], beta * nc / n, rtol=1e-12)
+    np.testing.assert_allclose(st["theta"], beta @ rows, rtol=1e-12)
+
+
+def test_norm_ratio_single_entry_closed_form(oracle):
+    """One complex entry x: c^T K c = ca^2 + cb^2 + 2 ca cb exp(-|x|^2 / var) because
+    |a - b|^2 = 2|x|^2; the ratio is wL |theta|^2 over wG times that."""
+    rng = np.random.default_rng(1)
+    N, wG, v = 4, 0.3, 0.2
+    o = oracle.Apsm(dict(windowSize=1, dictionarySize=4, gaussianKernelVariance=v, gaussianKernelWeight=wG), N)
+    x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    rows = _real_rows(x[:, None])
+    c = np.array([0.7, -0.4])
+    theta = rng.standard_normal(2 * N)
+    var = v * N * N
+    o.set_state(rows, c, theta, var)
+    q = c[0] ** 2 + c[1] ** 2 + 2 * c[0] * c[1] * np.exp(-np.sum(np.abs(x) ** 2) / var)
+    assert o.norm_ratio() == pytest.approx((1 - wG) * theta @ theta / (wG * q), rel=1e-12)
