@@ -126,33 +126,6 @@ struct StreamBuf
 
 constexpr float kLog2e = 1.4426950408889634f;
 
-// Pairwise-distance heuristic for the Gaussian variance (cpp/lib/apsm/apsm_noma_detector.cu:194-218),
-// same FP32 arithmetic so the percentile index matches.
-float varianceHeuristic( const float* rx, int N, int64_t T, const std::vector<int32_t>* perm )
-{
-    std::vector<float> nd;
-    nd.reserve( (size_t)( T * ( T - 1 ) / 2 ) );
-    for ( int64_t i = 0; i < T; i++ )
-        for ( int64_t j = i + 1; j < T; j++ )
-        {
-            int64_t pi = perm ? ( *perm )[ i ] : i, pj = perm ? ( *perm )[ j ] : j;
-            float dist = 0;
-            for ( int n = 0; n < N; n++ )
-            {
-                std::complex<float> a( rx[ ( n * T + pi ) * 2 ], rx[ ( n * T + pi ) * 2 + 1 ] );
-                std::complex<float> b( rx[ ( n * T + pj ) * 2 ], rx[ ( n * T + pj ) * 2 + 1 ] );
-                dist += std::abs( a - b );
-            }
-            nd.push_back( std::sqrt( dist ) );
-        }
-    if ( nd.empty() )
-        throw ApsmError { APSM_ERR_ARG, "variance heuristic needs at least two pilots" };
-    std::sort( nd.begin(), nd.end() );
-    const float p = 0.7f, q = 5.f;
-    float v = nd[ (size_t)( nd.size() * p ) ] * std::pow( (float)T, -1.f / q );
-    return v * v;
-}
-
 // Permutation of the reference shuffle (cpp/lib/noma/noma_detector.cu:59-74).
 std::vector<int32_t> shufflePermutation( int64_t T )
 {
@@ -537,10 +510,56 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
     }
 }
 
+// Pairwise-distance heuristic for the Gaussian variance (cpp/lib/apsm/apsm_noma_detector.cu:194-218)
+// on the device: the k-th smallest pair distance by a three-pass radix select, k as the
+// reference indexes its sorted vector (size * 0.7f in float). The set of pair distances does
+// not depend on the pilot order, so the training shuffle does not enter.
+float varianceHeuristicDevice( hipStream_t stream, const float* d_rx, int N, int64_t T )
+{
+    ARG_CHECK( T >= 2, "variance heuristic needs at least two pilots" );
+    const uint64_t P = (uint64_t)T * (uint64_t)( T - 1 ) / 2;
+    uint64_t k = (uint64_t)( (float)P * 0.7f );
+    if ( k >= P )
+        k = P - 1;
+    DevBuf<unsigned long long> hist;
+    hist.ensure( apsm::kVhBins );
+    std::vector<unsigned long long> hh( apsm::kVhBins );
+    const int tiles = gridFor( T, apsm::kVhTile );
+    ARG_CHECK( tiles <= 65535, "too many pilots for the variance heuristic" );
+    const size_t lds = 2 * (size_t)N * apsm::kVhTile * sizeof( float2 );
+    uint32_t prefix = 0;
+    for ( int pass = 0; pass < 3; pass++ )
+    {
+        HIP_OR_THROW( hipMemsetAsync( hist.ptr, 0, apsm::kVhBins * sizeof( unsigned long long ), stream ) );
+        hipLaunchKernelGGL( apsm::apsm_variance_histogram, dim3( tiles, tiles ), dim3( apsm::kVhThreads ), lds, stream, d_rx, N, T, pass,
+                            prefix, hist.ptr );
+        HIP_OR_THROW( hipGetLastError() );
+        HIP_OR_THROW( hipMemcpyAsync( hh.data(), hist.ptr, apsm::kVhBins * sizeof( unsigned long long ), hipMemcpyDeviceToHost, stream ) );
+        HIP_OR_THROW( hipStreamSynchronize( stream ) );
+        const int nb = pass == 2 ? 256 : apsm::kVhBins;
+        uint64_t cum = 0;
+        int b = 0;
+        for ( ; b < nb; b++ )
+        {
+            if ( cum + hh[ b ] > k )
+                break;
+            cum += hh[ b ];
+        }
+        if ( b == nb )
+            throw ApsmError { APSM_ERR_NUMERIC, "variance heuristic: percentile not found" };
+        k -= cum;
+        prefix = pass == 0 ? (uint32_t)b : ( prefix << ( pass == 1 ? 12 : 8 ) ) | (uint32_t)b;
+    }
+    float value;
+    std::memcpy( &value, &prefix, sizeof( value ) );
+    // float element times a double power, rounded to float, then squared in float
+    const float v = (float)( (double)value * std::pow( (double)T, -1. / 5. ) );
+    return v * v;
+}
+
 // Kernel variance (configured value scaled by N^2, or the heuristic on the call's pilots) and
 // the LLS initialisation of theta, both from the whole call (apsm_noma_detector.cu:96-105).
-void trainPrepare( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, const float* hostRx,
-                   const std::vector<int32_t>* perm )
+void trainPrepare( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T )
 {
     const apsm_params& p = h->p;
     const int N = h->N, L = h->L, U = h->U;
@@ -549,18 +568,7 @@ void trainPrepare( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t
         if ( p.gaussian_kernel_variance != 0.f )
             h->var = p.gaussian_kernel_variance * (float)( N * N );
         else
-        {
-            std::vector<float> tmp;
-            const float* src = hostRx;
-            if ( !src )
-            {
-                tmp.resize( (size_t)N * T * 2 );
-                HIP_OR_THROW( hipMemcpyAsync( tmp.data(), d_rx, tmp.size() * sizeof( float ), hipMemcpyDeviceToHost, h->stream ) );
-                HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
-                src = tmp.data();
-            }
-            h->var = varianceHeuristic( src, N, T, perm );
-        }
+            h->var = varianceHeuristicDevice( h->stream, d_rx, N, T );
         h->varSet = true;
     }
     const float wL = p.linear_kernel_weight;
@@ -591,7 +599,7 @@ void trainPrepare( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t
 
 // One train call. prepared: the caller already ran trainPrepare over the whole pilot set and
 // the rows arrive in final order (no shuffle here).
-void trainDev( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, const float* hostRx, bool prepared = false )
+void trainDev( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, bool prepared = false )
 {
     const apsm_params& p = h->p;
     ARG_CHECK( T > 0, "train needs at least one pilot" );
@@ -603,7 +611,7 @@ void trainDev( apsm_handle* h, const float* d_rx, const float* d_tx, int64_t T, 
     if ( shuffle )
         perm = shufflePermutation( T );
     if ( !prepared )
-        trainPrepare( h, d_rx, d_tx, T, hostRx, shuffle ? &perm : nullptr );
+        trainPrepare( h, d_rx, d_tx, T );
     const float wG = p.gaussian_kernel_weight, wL = p.linear_kernel_weight;
 
     // New stream buffer: the dictionary rows still live (at most Dc complex entries) + this call.
@@ -959,7 +967,7 @@ int apsm_train( apsm_handle* h, const float* rx, const float* tx, uint32_t T, fl
         h->tx.ensure( ntx );
         HIP_OR_THROW( hipMemcpyAsync( h->rx.ptr, rx, nrx * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
         HIP_OR_THROW( hipMemcpyAsync( h->tx.ptr, tx, ntx * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
-        trainDev( h, h->rx.ptr, h->tx.ptr, T, rx );
+        trainDev( h, h->rx.ptr, h->tx.ptr, T );
         float t = elapsedMs( h->ev0, h->ev1 );
         HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
         if ( ms )
@@ -971,7 +979,7 @@ int apsm_train_dev( apsm_handle* h, const float* d_rx, const float* d_tx, uint32
 {
     return guarded( [ & ] {
         ARG_CHECK( h && d_rx && d_tx, "null argument" );
-        trainDev( h, d_rx, d_tx, T, nullptr );
+        trainDev( h, d_rx, d_tx, T );
     } );
 }
 
@@ -1154,7 +1162,7 @@ int apsm_train_history( apsm_handle* h, const float* rx, const float* tx, uint32
         HIP_OR_THROW( hipMemcpyAsync( dTx.ptr, htx.data(), htx.size() * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
         HIP_OR_THROW( hipMemcpyAsync( dRv.ptr, rx_val, (size_t)N * T_val * 2 * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
         HIP_OR_THROW( hipMemcpyAsync( dTv.ptr, tx_val, (size_t)U * T_val * 2 * sizeof( float ), hipMemcpyHostToDevice, h->stream ) );
-        trainPrepare( h, dRx.ptr, dTx.ptr, T, hrx.data(), nullptr );
+        trainPrepare( h, dRx.ptr, dTx.ptr, T );
 
         hipEvent_t e0 = nullptr, e1 = nullptr;
         HIP_OR_THROW( hipEventCreate( &e0 ) );
@@ -1169,7 +1177,7 @@ int apsm_train_history( apsm_handle* h, const float* rx, const float* tx, uint32
                                             nC * 2 * sizeof( float ), N, hipMemcpyDeviceToDevice, h->stream ) );
             HIP_OR_THROW( hipMemcpy2DAsync( dChunkTx.ptr, nC * 2 * sizeof( float ), dTx.ptr + c0 * 2, T * 2 * sizeof( float ),
                                             nC * 2 * sizeof( float ), U, hipMemcpyDeviceToDevice, h->stream ) );
-            trainDev( h, dChunkRx.ptr, dChunkTx.ptr, nC, nullptr, true );
+            trainDev( h, dChunkRx.ptr, dChunkTx.ptr, nC, true );
             c0 += nC;
             // detect the whole training set and the validation set with the state after this step
             // (apsm_train.cu:147-158)

@@ -845,6 +845,81 @@ __global__ void __launch_bounds__( kMetricThreads ) apsm_history_metrics( const 
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Gaussian-variance heuristic (cpp/lib/apsm/apsm_noma_detector.cu:194-218): the 70th
+// percentile of d_ij = sqrt(sum_n |x_i[n] - x_j[n]|) over all pairs i < j of pilots. The
+// percentile is found without storing the T(T-1)/2 values: non-negative floats order as
+// their bit patterns, so three histogram passes over the pairs fix the top 12, the next 12
+// and the last 8 bits of the k-th smallest value. Tiles of 64 x 64 pilots sit in LDS;
+// per-workgroup histograms in LDS are added into a global 64-bit histogram.
+// ---------------------------------------------------------------------------------------
+constexpr int kVhTile = 64;
+constexpr int kVhThreads = 256;
+constexpr int kVhBins = 4096;
+
+__device__ __forceinline__ uint32_t vhBin( uint32_t bits, int pass, uint32_t prefix, bool& take )
+{
+    if ( pass == 0 )
+    {
+        take = true;
+        return bits >> 20;
+    }
+    if ( pass == 1 )
+    {
+        take = ( bits >> 20 ) == prefix;
+        return ( bits >> 8 ) & 0xFFFu;
+    }
+    take = ( bits >> 8 ) == prefix;
+    return bits & 0xFFu;
+}
+
+__global__ void __launch_bounds__( kVhThreads ) apsm_variance_histogram( const float* __restrict__ rx, int N, int64_t T, int pass,
+                                                                        uint32_t prefix, unsigned long long* __restrict__ hist )
+{
+    const int bi = blockIdx.y, bj = blockIdx.x;
+    if ( bj < bi )
+        return;
+    extern __shared__ float sv[]; // [2][N][kVhTile] complex as float2
+    float2* sa = reinterpret_cast<float2*>( sv );
+    float2* sb = sa + N * kVhTile;
+    __shared__ uint32_t h[ kVhBins ];
+    for ( int b = threadIdx.x; b < kVhBins; b += kVhThreads )
+        h[ b ] = 0u;
+    const int64_t i0 = (int64_t)bi * kVhTile, j0 = (int64_t)bj * kVhTile;
+    for ( int e = threadIdx.x; e < N * kVhTile; e += kVhThreads )
+    {
+        const int n = e / kVhTile, k = e % kVhTile;
+        const float2* r = reinterpret_cast<const float2*>( rx ) + (int64_t)n * T;
+        sa[ e ] = i0 + k < T ? r[ i0 + k ] : make_float2( 0.f, 0.f );
+        sb[ e ] = j0 + k < T ? r[ j0 + k ] : make_float2( 0.f, 0.f );
+    }
+    __syncthreads();
+    const int il = threadIdx.x % kVhTile;
+    const int jb = ( threadIdx.x / kVhTile ) * ( kVhTile * kVhTile / kVhThreads );
+    const int64_t i = i0 + il;
+    for ( int jj = 0; jj < kVhTile * kVhTile / kVhThreads; jj++ )
+    {
+        const int jl = jb + jj;
+        const int64_t j = j0 + jl;
+        if ( i >= T || j >= T || j <= i )
+            continue;
+        float dist = 0.f;
+        for ( int n = 0; n < N; n++ )
+        {
+            const float2 a = sa[ n * kVhTile + il ], b = sb[ n * kVhTile + jl ];
+            dist += hypotf( a.x - b.x, a.y - b.y );
+        }
+        bool take;
+        const uint32_t bin = vhBin( __float_as_uint( sqrtf( dist ) ), pass, prefix, take );
+        if ( take )
+            atomicAdd( &h[ bin ], 1u );
+    }
+    __syncthreads();
+    for ( int b = threadIdx.x; b < kVhBins; b += kVhThreads )
+        if ( h[ b ] )
+            atomicAdd( &hist[ b ], (unsigned long long)h[ b ] );
+}
+
 __global__ void fill_zero( float* __restrict__ p, int64_t n )
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -316,7 +316,8 @@ float varianceHeuristic( const float* rx, int N, int64_t T, const std::vector<in
         }
     std::sort( nd.begin(), nd.end() );
     const float p = 0.7f, q = 5.f;
-    float v = nd[ (size_t)( nd.size() * p ) ] * std::pow( (float)T, -1.f / q );
+    // float element times a double power, rounded to float, then squared in float
+    float v = (float)( (double)nd[ (size_t)( nd.size() * p ) ] * std::pow( (double)T, -1. / q ) );
     return v * v;
 }
 

@@ -161,3 +161,18 @@ def test_train_history_rejects_passes(require_gpu):
     det = ApsmNomaDetectorFactory.build(dict(otype="APSM", windowSize=4, dictionarySize=40, trainPasses=2), 4)
     with pytest.raises(ApsmError):
         det.train(d["rx_train"], d["tx_train"][0], (NomaTrainHistory(2, 0.7071), d["rx_data"], d["tx_data"][0]))
+
+
+@pytest.mark.parametrize("N,T,shuffle", [(4, 2, False), (4, 301, False), (8, 3000, True), (16, 1200, False)])
+def test_variance_heuristic_on_device(require_gpu, oracle, N, T, shuffle):
+    """gaussianKernelVariance = 0 selects the pair-distance heuristic; the device's radix select
+    returns the oracle's sorted-vector element (within the last-bit differences of |a - b|)."""
+    from libapsm_amd import ApsmNomaDetectorFactory
+    d = small_problem(N=N, T=T, M=10, seed=T)
+    det = ApsmNomaDetectorFactory.build(dict(otype="APSM", windowSize=1, dictionarySize=T, gaussianKernelVariance=0.0,
+                                             trainingShuffle=shuffle), N)
+    det.train(d["rx_train"], d["tx_train"][0])
+    got = det.getTrainingState()["variance"]
+    ref = oracle.variance_heuristic(d["rx_train"])
+    print(f"variance heuristic gpu {got:.9g} oracle {ref:.9g}")
+    assert got == pytest.approx(ref, rel=2e-6)
