@@ -181,6 +181,33 @@ float* lls_weights_dev(lls_handle* h);
 /* The handle's stream (hipStream_t), for callers that order their own work after it. */
 void* lls_stream(lls_handle* h);
 
+/* Batched instances (no counterpart in the reference, which runs one detector per host
+ * thread, cpp/cli/detect/noma_detect.cpp:100-121): n independent detectors, each with its own
+ * parameters, antenna count and users, run as one launch sequence. A run is, for every
+ * instance: reset, train on its pilots, detect its data samples -- the same results as
+ * apsm_reset + apsm_train_dev + apsm_detect_dev on a handle of its own.
+ * apsm_batch_plan fixes the inputs (device pointers in the *_dev layouts, one per instance),
+ * builds the launch plan, picks the detect launch shape per (antennas, users) bucket by timing
+ * candidates on these inputs, and (use_graph != 0) captures the run in a hipGraph; it leaves
+ * one completed run behind. apsm_batch_run enqueues a run (graph replay) on the batch stream.
+ * Instances need the blocked train path (trainPasses 1, windowSize <= 160, sampleStep 2 or 4,
+ * no startWithFullWindow, no normConstraint), a non-zero gaussianKernelVariance, no LLS
+ * initialisation or shuffle, and at most 16 users (APSM_ERR_UNSUPPORTED otherwise). */
+typedef struct apsm_batch apsm_batch;
+int apsm_batch_create(uint32_t n, const apsm_params* params, const uint32_t* num_antennas, const uint32_t* num_users,
+                      apsm_batch** out);
+void apsm_batch_destroy(apsm_batch* b);
+int apsm_batch_plan(apsm_batch* b, const float* const* d_rx_train, const float* const* d_tx_train, const uint32_t* T,
+                    const float* const* d_rx_data, const uint32_t* M, float* const* d_est, int use_graph);
+int apsm_batch_run(apsm_batch* b);
+int apsm_batch_synchronize(apsm_batch* b);
+/* Device time of the most recent run (HIP events around it on the batch stream), in ms. */
+int apsm_batch_last_ms(apsm_batch* b, float* ms);
+/* Instance i's handle (state readback with apsm_get_state; owned by the batch). */
+apsm_handle* apsm_batch_instance(apsm_batch* b, uint32_t i);
+/* Autotuner choices: rows of (NMAX, UMAX, threads per workgroup, workgroups per CU, work items). */
+int apsm_batch_tuning(apsm_batch* b, int32_t* table, uint32_t cap_rows, uint32_t* rows);
+
 /* CudaStream (cpp/lib/cuda/cuda_stream.cuh:67-96): a HIP stream of default priority. */
 int apsm_stream_create(void** out);
 int apsm_stream_destroy(void* stream);

@@ -81,6 +81,14 @@ SIGNATURES = {
     "apsm_memcpy_h2d": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
     "apsm_memcpy_d2h": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
     "apsm_device_synchronize": (ctypes.c_int, []),
+    "apsm_batch_create": (ctypes.c_int, [_u32, _P(ApsmParamsC), _P(_u32), _P(_u32), _P(_vp)]),
+    "apsm_batch_destroy": (None, [_vp]),
+    "apsm_batch_plan": (ctypes.c_int, [_vp, _P(_vp), _P(_vp), _P(_u32), _P(_vp), _P(_u32), _P(_vp), ctypes.c_int]),
+    "apsm_batch_run": (ctypes.c_int, [_vp]),
+    "apsm_batch_synchronize": (ctypes.c_int, [_vp]),
+    "apsm_batch_last_ms": (ctypes.c_int, [_vp, _f32p]),
+    "apsm_batch_instance": (_vp, [_vp, _u32]),
+    "apsm_batch_tuning": (ctypes.c_int, [_vp, _P(ctypes.c_int32), _u32, _P(_u32)]),
     "apsm_stream_create": (ctypes.c_int, [_P(_vp)]),
     "apsm_stream_destroy": (ctypes.c_int, [_vp]),
     "apsm_stream_synchronize": (ctypes.c_int, [_vp]),

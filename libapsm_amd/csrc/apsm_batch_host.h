@@ -1,0 +1,601 @@
+// Host side of the batched path (included by apsm_capi.hip after the handle definitions).
+//
+// A batch owns one detector handle per instance. A run is, for every instance at once: reset,
+// train on its pilots, detect its data samples. Planning walks every instance's train schedule
+// (the same blocks trainBlocked forms), groups the kernels of all instances by round and by
+// template instance, writes one descriptor per (launch, instance) into a single device buffer,
+// picks the detect launch shape per bucket by timing candidates on the real batch, and records
+// the launch sequence. A run replays it, captured once into a hipGraph, on the batch's stream.
+#pragma once
+
+#include <map>
+#include <tuple>
+
+struct apsm_batch
+{
+    std::vector<apsm_handle*> inst;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool planned = false, useGraph = true, timed = false;
+    DevBuf<char> desc; // every descriptor array of the plan
+    std::vector<std::function<void()>> launches; // the run, in order (kernel launches only)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<std::array<int, 5>> tuning; // (NMAX, UMAX, threads, blocks per CU, items)
+    int cus = 256;
+};
+
+namespace {
+
+// Detect launch shapes the autotuner chooses from: threads per workgroup x workgroups per CU.
+constexpr int kTuneThreads[] = { 128, 256, 512 };
+constexpr int kTuneBlocksPerCu[] = { 2, 4, 8 };
+
+template <int NMAX, int UMAX, int THREADS>
+void launchBatchDetect( hipStream_t s, const apsm::DetDesc* d, const apsm::WorkItem* items, int nItems, int grid )
+{
+    hipLaunchKernelGGL( ( apsm::batch_detect<NMAX, UMAX, THREADS> ), dim3( grid ), dim3( THREADS ), 0, s, d, items, nItems );
+    HIP_OR_THROW( hipGetLastError() );
+}
+
+template <int NMAX, int UMAX>
+void launchBatchDetectT( hipStream_t s, int threads, const apsm::DetDesc* d, const apsm::WorkItem* items, int nItems, int grid )
+{
+    switch ( threads )
+    {
+    case 128: launchBatchDetect<NMAX, UMAX, 128>( s, d, items, nItems, grid ); break;
+    case 256: launchBatchDetect<NMAX, UMAX, 256>( s, d, items, nItems, grid ); break;
+    default: launchBatchDetect<NMAX, UMAX, 512>( s, d, items, nItems, grid ); break;
+    }
+}
+
+template <int NMAX>
+void launchBatchDetectU( hipStream_t s, int umax, int threads, const apsm::DetDesc* d, const apsm::WorkItem* items, int nItems, int grid )
+{
+    switch ( umax )
+    {
+    case 1: launchBatchDetectT<NMAX, 1>( s, threads, d, items, nItems, grid ); break;
+    case 2: launchBatchDetectT<NMAX, 2>( s, threads, d, items, nItems, grid ); break;
+    case 4: launchBatchDetectT<NMAX, 4>( s, threads, d, items, nItems, grid ); break;
+    case 8: launchBatchDetectT<NMAX, 8>( s, threads, d, items, nItems, grid ); break;
+    default: launchBatchDetectT<NMAX, 16>( s, threads, d, items, nItems, grid ); break;
+    }
+}
+
+void launchBatchDetectAny( hipStream_t s, int nmax, int umax, int threads, const apsm::DetDesc* d, const apsm::WorkItem* items,
+                           int nItems, int grid )
+{
+    switch ( nmax )
+    {
+    case 4: launchBatchDetectU<4>( s, umax, threads, d, items, nItems, grid ); break;
+    case 8: launchBatchDetectU<8>( s, umax, threads, d, items, nItems, grid ); break;
+    case 16: launchBatchDetectU<16>( s, umax, threads, d, items, nItems, grid ); break;
+    case 32: launchBatchDetectU<32>( s, umax, threads, d, items, nItems, grid ); break;
+    default: launchBatchDetectU<64>( s, umax, threads, d, items, nItems, grid ); break;
+    }
+}
+
+template <int UMAX>
+void launchBatchEntry( hipStream_t s, const apsm::EntryDesc* d, int n, int gridX )
+{
+    hipLaunchKernelGGL( ( apsm::batch_entry<UMAX> ), dim3( gridX, n ), dim3( apsm::kEntryThreads ), 0, s, d );
+    HIP_OR_THROW( hipGetLastError() );
+}
+
+void launchBatchEntryAny( hipStream_t s, int umax, const apsm::EntryDesc* d, int n, int gridX )
+{
+    switch ( umax )
+    {
+    case 1: launchBatchEntry<1>( s, d, n, gridX ); break;
+    case 2: launchBatchEntry<2>( s, d, n, gridX ); break;
+    case 4: launchBatchEntry<4>( s, d, n, gridX ); break;
+    case 8: launchBatchEntry<8>( s, d, n, gridX ); break;
+    default: launchBatchEntry<16>( s, d, n, gridX ); break;
+    }
+}
+
+template <int TP, int NEWPER, int PMAX>
+void launchBatchSeq2( hipStream_t s, const apsm::Seq2Desc* d, int n, int gridX, size_t lds )
+{
+    if ( lds > 64 * 1024 )
+        HIP_OR_THROW( hipFuncSetAttribute( (const void*)apsm::batch_seq2<TP, NEWPER, PMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds ) );
+    hipLaunchKernelGGL( ( apsm::batch_seq2<TP, NEWPER, PMAX> ), dim3( gridX, n ), dim3( apsm::kSeqThreads ), lds, s, d );
+    HIP_OR_THROW( hipGetLastError() );
+}
+
+template <int NEWPER>
+void launchBatchSeq2Tp( hipStream_t s, int tp, const apsm::Seq2Desc* d, int n, int gridX, size_t lds )
+{
+    switch ( tp )
+    {
+    case 2: launchBatchSeq2<2, NEWPER, 24>( s, d, n, gridX, lds ); break;
+    case 4: launchBatchSeq2<4, NEWPER, 16>( s, d, n, gridX, lds ); break;
+    case 7: launchBatchSeq2<7, NEWPER, 12>( s, d, n, gridX, lds ); break;
+    case 10: launchBatchSeq2<10, NEWPER, 10>( s, d, n, gridX, lds ); break;
+    case 16: launchBatchSeq2<16, NEWPER, 8>( s, d, n, gridX, lds ); break;
+    default: launchBatchSeq2<27, NEWPER, 6>( s, d, n, gridX, lds ); break;
+    }
+}
+
+int pmaxOfTp( int tp )
+{
+    switch ( tp )
+    {
+    case 2: return 24;
+    case 4: return 16;
+    case 7: return 12;
+    case 10: return 10;
+    case 16: return 8;
+    default: return 6;
+    }
+}
+
+int roundUpPow2Users( int U ) { return U <= 1 ? 1 : U <= 2 ? 2 : U <= 4 ? 4 : U <= 8 ? 8 : 16; }
+int nmaxOf( int N ) { return N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64; }
+
+// Host-side descriptor storage: typed arrays appended into one byte buffer; launches refer to
+// offsets that become device pointers once the buffer is uploaded.
+struct DescArena
+{
+    std::vector<char> bytes;
+    template <typename T>
+    size_t add( const std::vector<T>& v )
+    {
+        size_t off = ( bytes.size() + 255 ) & ~size_t( 255 );
+        bytes.resize( off + v.size() * sizeof( T ) );
+        std::memcpy( bytes.data() + off, v.data(), v.size() * sizeof( T ) );
+        return off;
+    }
+};
+
+void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT, const uint32_t* T, const float* const* rxD, const uint32_t* M,
+                float* const* est, bool useGraph )
+{
+    const int n = (int)B->inst.size();
+    ARG_CHECK( n > 0, "empty batch" );
+    for ( int i = 0; i < n; i++ )
+    {
+        apsm_handle* h = B->inst[ i ];
+        const apsm_params& p = h->p;
+        ARG_CHECK( rxT[ i ] && txT[ i ] && rxD[ i ] && est[ i ] && T[ i ] > 0 && M[ i ] > 0, "null or empty batch input" );
+        if ( !h->blocked || p.gaussian_kernel_variance == 0.f || p.lls_initialization || p.training_shuffle || h->U > 16 )
+            throw ApsmError { APSM_ERR_UNSUPPORTED,
+                              "batched instances need the blocked train path, a configured kernel variance, no LLS initialisation, "
+                              "no shuffle and at most 16 users" };
+    }
+    if ( B->exec )
+    {
+        (void)hipGraphExecDestroy( B->exec );
+        B->exec = nullptr;
+    }
+    if ( B->graph )
+    {
+        (void)hipGraphDestroy( B->graph );
+        B->graph = nullptr;
+    }
+    B->launches.clear();
+    B->tuning.clear();
+
+    DescArena arena;
+    // per-run reset: f, theta share and theta of every instance (fill kernels, capturable)
+    std::vector<float*> zp;
+    std::vector<int64_t> zc;
+    std::vector<apsm::StageDesc> stage;
+    int64_t maxS = 0;
+    struct Blk
+    {
+        apsm::EntryDesc e;
+        apsm::GramDesc g;
+        apsm::EvDesc v;
+        apsm::Seq2Desc s;
+        bool hasEv;
+        int umax, tp, newPer;
+        size_t lds;
+    };
+    std::vector<std::vector<Blk>> blocks( n );
+    std::vector<apsm::ThetaDesc> theta( n );
+    std::vector<apsm::DetDesc> det( n );
+    for ( int i = 0; i < n; i++ )
+    {
+        apsm_handle* h = B->inst[ i ];
+        const apsm_params& p = h->p;
+        const int N = h->N, L = h->L, U = h->U, w2 = h->w2, Wc = w2 / 2;
+        const int64_t Ti = T[ i ];
+        // fresh state: the stream buffer holds this call's pilots only
+        h->varSet = true;
+        h->var = p.gaussian_kernel_variance * (float)( N * N );
+        h->anyTrain = true;
+        StreamBuf& b = h->buf[ 0 ];
+        b.ensure( Ti, N, U );
+        const int64_t ld = 2 * b.capK;
+        zp.push_back( h->F.ptr );
+        zc.push_back( (int64_t)U * w2 );
+        zp.push_back( h->thOld.ptr );
+        zc.push_back( (int64_t)U * L );
+        zp.push_back( h->theta.ptr );
+        zc.push_back( (int64_t)U * L );
+        stage.push_back( apsm::StageDesc { rxT[ i ], txT[ i ], b.Z.ptr, b.Zn.ptr, b.Y.ptr, b.C.ptr, Ti, ld, N, U } );
+        maxS = std::max( maxS, Ti );
+
+        const float wG = p.gaussian_kernel_weight, wL = p.linear_kernel_weight;
+        const float g2 = 0.5f / h->var * kLog2e;
+        const int64_t q = p.sample_step;
+        const int newPer = (int)( q / 2 );
+        const int64_t R2 = 2 * Ti;
+        const int64_t nStepsTotal = R2 / q;
+        int64_t steps = std::max<int64_t>( 1, std::min<int64_t>( kBlockRowsMax / q, ( h->H - w2 ) / q + 1 ) );
+        while ( steps > 1 && ( w2 + steps * q ) / 2 > apsm::kActMax )
+            steps--;
+        if ( ( w2 + steps * q ) / 2 > apsm::kActMax )
+            throw ApsmError { APSM_ERR_UNSUPPORTED, "window too large for the blocked train path" };
+        h->Pbuf.ensure( (size_t)U * steps * newPer * 2 );
+        h->Gact.ensure( (size_t)apsm::triIdx( ( w2 + steps * q ) / 2 ) * 3 );
+        h->EvF.ensure( (size_t)U * steps * w2 );
+        const int tp = seq2Tp( Wc );
+        const int nParts = std::min( apsm::kSeqThreads / Wc, ( Wc + tp - 1 ) / tp );
+        if ( tp < 0 || nParts > apsm::kMaxParts || nParts * tp < Wc || nParts > pmaxOfTp( tp ) )
+            throw ApsmError { APSM_ERR_UNSUPPORTED, "window layout does not fit the blocked train path" };
+        int64_t a0Next = 0;
+        for ( int64_t k0 = 0; k0 < nStepsTotal; k0 += steps )
+        {
+            const int64_t nb = std::min( steps, nStepsTotal - k0 );
+            const int64_t sPrev = k0 * q, sFirst = sPrev + q, sLast = sPrev + nb * q;
+            const int64_t a0 = sFirst - std::min<int64_t>( w2, sFirst );
+            const int64_t sNext = sLast + q;
+            a0Next = sNext - std::min<int64_t>( w2, sNext );
+            const int nNewC = (int)( nb * newPer );
+            const int nActC = (int)( ( sLast - a0 ) / 2 );
+            Blk k {};
+            k.umax = roundUpPow2Users( U );
+            k.tp = tp;
+            k.newPer = newPer;
+            apsm::EntryArgs& ea = k.e.a;
+            ea.N = N;
+            ea.L = L;
+            ea.U = U;
+            ea.q = (int)q;
+            ea.rowBase = 0;
+            ea.newC0 = sPrev / 2;
+            ea.sFirst = sFirst;
+            ea.H = h->H;
+            ea.a0 = a0;
+            ea.ldYC = ld;
+            ea.nNewC = nNewC;
+            ea.wL = wL;
+            ea.wG = wG;
+            ea.g2 = g2;
+            k.e.Z = b.Z.ptr;
+            k.e.Zn = b.Zn.ptr;
+            k.e.C = b.C.ptr;
+            k.e.thF = h->thOld.ptr;
+            k.e.P = h->Pbuf.ptr;
+            apsm::GramArgs& ga = k.g.a;
+            ga.N = N;
+            ga.L = L;
+            ga.c0 = a0 / 2;
+            ga.nA = nActC;
+            ga.wL = wL;
+            ga.wG = wG;
+            ga.g2 = g2;
+            k.g.Z = b.Z.ptr;
+            k.g.Zn = b.Zn.ptr;
+            k.g.Gact = h->Gact.ptr;
+            const int64_t loLast = sLast > h->H ? sLast - h->H : 0, loPrev = sPrev > h->H ? sPrev - h->H : 0;
+            k.hasEv = loLast > loPrev;
+            apsm::EvArgs& ev = k.v.a;
+            ev.N = N;
+            ev.L = L;
+            ev.U = U;
+            ev.q = (int)q;
+            ev.w2 = w2;
+            ev.nSteps = (int)nb;
+            ev.rowBase = 0;
+            ev.sPrev = sPrev;
+            ev.sFirst = sFirst;
+            ev.H = h->H;
+            ev.ldYC = ld;
+            ev.wG = wG;
+            ev.g2 = g2;
+            k.v.Z = b.Z.ptr;
+            k.v.Zn = b.Zn.ptr;
+            k.v.C = b.C.ptr;
+            k.v.EvF = h->EvF.ptr;
+            apsm::Seq2Args& s2 = k.s.a;
+            s2.N = N;
+            s2.L = L;
+            s2.U = U;
+            s2.w2 = w2;
+            s2.Wc = Wc;
+            s2.q = (int)q;
+            s2.nSteps = (int)nb;
+            s2.hasEv = k.hasEv ? 1 : 0;
+            s2.nParts = nParts;
+            s2.ramp = a0 == 0 ? 1 : 0;
+            s2.a0modW = (int)( a0 % w2 );
+            s2.sFirstRel = (int)( sFirst - a0 );
+            s2.nNewC = nNewC;
+            s2.nActC = nActC;
+            s2.newRel0 = (int)( ( sPrev - a0 ) / 2 );
+            s2.nFr = (int)( ( a0Next - a0 ) / 2 );
+            s2.pkFloats = newPer * 3 * nActC + 2 * newPer + w2;
+            s2.ldYC = ld;
+            s2.wL = wL;
+            s2.wG = wG;
+            s2.eB = p.eB;
+            k.s.Z = b.Z.ptr + ( a0 / 2 ) * L;
+            k.s.Zn = b.Zn.ptr + a0 / 2;
+            k.s.Y = b.Y.ptr + a0;
+            k.s.C = b.C.ptr + a0;
+            k.s.Fg = h->F.ptr;
+            k.s.thF = h->thOld.ptr;
+            k.s.P = h->Pbuf.ptr;
+            k.s.Gact = h->Gact.ptr;
+            k.s.EvF = h->EvF.ptr;
+            k.lds = apsm::seq2LdsBytes( newPer, w2, tp );
+            blocks[ i ].push_back( k );
+        }
+        // stream bookkeeping after the train (same schedule as the kernels)
+        const int64_t used = nStepsTotal * q;
+        h->used = (uint64_t)used;
+        h->a0Next = a0Next;
+        h->cur = 0;
+        h->bufBaseC = 0;
+        h->bufK = Ti;
+        h->trainTimed = false;
+        const int64_t newUsedC = used / 2;
+        const int64_t lo = std::min<int64_t>( a0Next / 2, newUsedC );
+        theta[ i ] = apsm::ThetaDesc { b.Z.ptr, b.C.ptr, h->thOld.ptr, h->theta.ptr, lo, newUsedC, ld, N, U };
+        const int64_t dlo = std::max<int64_t>( 0, newUsedC - h->Hc );
+        const int64_t G = newUsedC - dlo;
+        det[ i ] = apsm::DetDesc { rxD[ i ],
+                                   G > 0 ? b.Z.ptr + dlo * L : nullptr,
+                                   G > 0 ? b.Zn.ptr + dlo : nullptr,
+                                   G > 0 ? b.C.ptr + 2 * dlo : nullptr,
+                                   h->theta.ptr,
+                                   est[ i ],
+                                   (int64_t)M[ i ],
+                                   ld,
+                                   G,
+                                   U,
+                                   N,
+                                   p.linear_kernel_weight,
+                                   p.gaussian_kernel_weight,
+                                   g2 };
+    }
+
+    // ---- launch list ----
+    struct Pending
+    {
+        size_t off;
+        std::function<void( const char* base )> fn;
+    };
+    std::vector<Pending> pend;
+    const size_t zpOff = arena.add( zp ), zcOff = arena.add( zc );
+    const int nz = (int)zp.size();
+    pend.push_back( { 0, [ = ]( const char* base ) {
+                         hipLaunchKernelGGL( apsm::batch_fill_zero, dim3( 4, nz ), dim3( 256 ), 0, B->stream, (float* const*)( base + zpOff ),
+                                             (const int64_t*)( base + zcOff ) );
+                         HIP_OR_THROW( hipGetLastError() );
+                     } } );
+    {
+        const size_t off = arena.add( stage );
+        const int ns = (int)stage.size(), gx = gridFor( maxS, 256 );
+        pend.push_back( { 0, [ = ]( const char* base ) {
+                             hipLaunchKernelGGL( apsm::batch_stage, dim3( gx, ns ), dim3( 256 ), 0, B->stream,
+                                                 (const apsm::StageDesc*)( base + off ) );
+                             HIP_OR_THROW( hipGetLastError() );
+                         } } );
+    }
+    size_t maxBlocks = 0;
+    for ( auto& v : blocks )
+        maxBlocks = std::max( maxBlocks, v.size() );
+    for ( size_t r = 0; r < maxBlocks; r++ )
+    {
+        std::map<int, std::vector<apsm::EntryDesc>> entry;
+        std::map<int, int> entryX;
+        std::vector<apsm::GramDesc> gram;
+        int64_t maxPairs = 0;
+        std::vector<apsm::EvDesc> evict;
+        int64_t maxEv = 0;
+        std::map<std::pair<int, int>, std::vector<apsm::Seq2Desc>> seq;
+        std::map<std::pair<int, int>, std::pair<int, size_t>> seqDim; // (max users, max lds)
+        for ( int i = 0; i < n; i++ )
+        {
+            if ( r >= blocks[ i ].size() )
+                continue;
+            const Blk& k = blocks[ i ][ r ];
+            entry[ k.umax ].push_back( k.e );
+            entryX[ k.umax ] = std::max( entryX[ k.umax ], k.e.a.nNewC );
+            gram.push_back( k.g );
+            maxPairs = std::max<int64_t>( maxPairs, apsm::triIdx( k.g.a.nA ) );
+            if ( k.hasEv )
+            {
+                evict.push_back( k.v );
+                maxEv = std::max<int64_t>( maxEv, (int64_t)k.v.a.nSteps * ( k.v.a.w2 / 2 ) );
+            }
+            const auto key = std::make_pair( k.tp, k.newPer );
+            seq[ key ].push_back( k.s );
+            auto& dim = seqDim[ key ];
+            dim.first = std::max( dim.first, k.s.a.U );
+            dim.second = std::max( dim.second, k.lds );
+        }
+        for ( auto& kv : entry )
+        {
+            const size_t off = arena.add( kv.second );
+            const int cnt = (int)kv.second.size(), umax = kv.first, gx = entryX[ kv.first ];
+            pend.push_back( { 0, [ = ]( const char* base ) {
+                                 launchBatchEntryAny( B->stream, umax, (const apsm::EntryDesc*)( base + off ), cnt, gx );
+                             } } );
+        }
+        if ( !gram.empty() )
+        {
+            const size_t off = arena.add( gram );
+            const int cnt = (int)gram.size(), gx = gridFor( maxPairs, 256 );
+            pend.push_back( { 0, [ = ]( const char* base ) {
+                                 hipLaunchKernelGGL( apsm::batch_gram, dim3( gx, cnt ), dim3( 256 ), 0, B->stream,
+                                                     (const apsm::GramDesc*)( base + off ) );
+                                 HIP_OR_THROW( hipGetLastError() );
+                             } } );
+        }
+        if ( !evict.empty() )
+        {
+            const size_t off = arena.add( evict );
+            const int cnt = (int)evict.size(), gx = gridFor( maxEv, 256 );
+            pend.push_back( { 0, [ = ]( const char* base ) {
+                                 hipLaunchKernelGGL( apsm::batch_evict, dim3( gx, cnt ), dim3( 256 ), 0, B->stream,
+                                                     (const apsm::EvDesc*)( base + off ) );
+                                 HIP_OR_THROW( hipGetLastError() );
+                             } } );
+        }
+        for ( auto& kv : seq )
+        {
+            const size_t off = arena.add( kv.second );
+            const int cnt = (int)kv.second.size(), tp = kv.first.first, newPer = kv.first.second;
+            const int gx = seqDim[ kv.first ].first;
+            const size_t lds = seqDim[ kv.first ].second;
+            pend.push_back( { 0, [ = ]( const char* base ) {
+                                 const apsm::Seq2Desc* d = (const apsm::Seq2Desc*)( base + off );
+                                 if ( newPer == 1 )
+                                     launchBatchSeq2Tp<1>( B->stream, tp, d, cnt, gx, lds );
+                                 else
+                                     launchBatchSeq2Tp<2>( B->stream, tp, d, cnt, gx, lds );
+                             } } );
+        }
+    }
+    {
+        int maxU = 1;
+        for ( auto& t : theta )
+            maxU = std::max( maxU, t.U );
+        const size_t off = arena.add( theta );
+        pend.push_back( { 0, [ = ]( const char* base ) {
+                             hipLaunchKernelGGL( apsm::batch_theta, dim3( maxU, n ), dim3( 256 ), 0, B->stream,
+                                                 (const apsm::ThetaDesc*)( base + off ) );
+                             HIP_OR_THROW( hipGetLastError() );
+                         } } );
+    }
+    const size_t trainLaunches = pend.size();
+
+    // detect buckets by (NMAX, UMAX); work items for every candidate tile size
+    std::map<std::pair<int, int>, std::vector<int>> dBuckets;
+    for ( int i = 0; i < n; i++ )
+        dBuckets[ { nmaxOf( det[ i ].N ), roundUpPow2Users( det[ i ].U ) } ].push_back( i );
+    const size_t detOff = arena.add( det );
+    struct DetBucket
+    {
+        int nmax, umax;
+        size_t itemOff[ 3 ];
+        int nItems[ 3 ];
+    };
+    std::vector<DetBucket> dbs;
+    for ( auto& kv : dBuckets )
+    {
+        DetBucket db {};
+        db.nmax = kv.first.first;
+        db.umax = kv.first.second;
+        for ( int c = 0; c < 3; c++ )
+        {
+            std::vector<apsm::WorkItem> items;
+            for ( int i : kv.second )
+                for ( int64_t t = 0; t < ( det[ i ].M + kTuneThreads[ c ] - 1 ) / kTuneThreads[ c ]; t++ )
+                    items.push_back( apsm::WorkItem { i, (int)t } );
+            db.itemOff[ c ] = arena.add( items );
+            db.nItems[ c ] = (int)items.size();
+        }
+        dbs.push_back( db );
+    }
+
+    // upload the descriptors
+    B->desc.ensure( arena.bytes.size() );
+    HIP_OR_THROW( hipMemcpy( B->desc.ptr, arena.bytes.data(), arena.bytes.size(), hipMemcpyHostToDevice ) );
+    const char* base = B->desc.ptr;
+    for ( auto& pd : pend )
+    {
+        auto fn = pd.fn;
+        B->launches.push_back( [ fn, base ]() { fn( base ); } );
+    }
+
+    // run the train once eagerly (the detect candidates are timed on the trained state)
+    for ( size_t l = 0; l < trainLaunches; l++ )
+        B->launches[ l ]();
+    HIP_OR_THROW( hipStreamSynchronize( B->stream ) );
+
+    // autotune: every (threads, workgroups per CU) candidate per bucket, best of two runs
+    hipEvent_t t0, t1;
+    HIP_OR_THROW( hipEventCreate( &t0 ) );
+    HIP_OR_THROW( hipEventCreate( &t1 ) );
+    for ( auto& db : dbs )
+    {
+        float best = 1e30f;
+        int bc = 1, bb = 4;
+        for ( int c = 0; c < 3; c++ )
+            for ( int bpc : kTuneBlocksPerCu )
+            {
+                const int grid = std::max( 1, std::min( db.nItems[ c ], B->cus * bpc ) );
+                float ms = 1e30f;
+                for ( int rep = 0; rep < 2; rep++ )
+                {
+                    HIP_OR_THROW( hipEventRecord( t0, B->stream ) );
+                    launchBatchDetectAny( B->stream, db.nmax, db.umax, kTuneThreads[ c ], (const apsm::DetDesc*)( base + detOff ),
+                                          (const apsm::WorkItem*)( base + db.itemOff[ c ] ), db.nItems[ c ], grid );
+                    HIP_OR_THROW( hipEventRecord( t1, B->stream ) );
+                    HIP_OR_THROW( hipEventSynchronize( t1 ) );
+                    float t = 0.f;
+                    HIP_OR_THROW( hipEventElapsedTime( &t, t0, t1 ) );
+                    ms = std::min( ms, t );
+                }
+                if ( ms < best )
+                {
+                    best = ms;
+                    bc = c;
+                    bb = bpc;
+                }
+            }
+        const int threads = kTuneThreads[ bc ], nItems = db.nItems[ bc ];
+        const int grid = std::max( 1, std::min( nItems, B->cus * bb ) );
+        const apsm::DetDesc* dd = (const apsm::DetDesc*)( base + detOff );
+        const apsm::WorkItem* it = (const apsm::WorkItem*)( base + db.itemOff[ bc ] );
+        const int nmax = db.nmax, umax = db.umax;
+        B->launches.push_back( [ = ]() { launchBatchDetectAny( B->stream, nmax, umax, threads, dd, it, nItems, grid ); } );
+        B->tuning.push_back( { nmax, umax, threads, bb, nItems } );
+    }
+    (void)hipEventDestroy( t0 );
+    (void)hipEventDestroy( t1 );
+
+    B->useGraph = useGraph;
+    if ( useGraph )
+    {
+        HIP_OR_THROW( hipStreamBeginCapture( B->stream, hipStreamCaptureModeRelaxed ) );
+        try
+        {
+            for ( auto& l : B->launches )
+                l();
+        }
+        catch ( ... )
+        {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture( B->stream, &g );
+            if ( g )
+                (void)hipGraphDestroy( g );
+            throw;
+        }
+        HIP_OR_THROW( hipStreamEndCapture( B->stream, &B->graph ) );
+        HIP_OR_THROW( hipGraphInstantiate( &B->exec, B->graph, nullptr, nullptr, 0 ) );
+    }
+    B->planned = true;
+}
+
+void batchRun( apsm_batch* B )
+{
+    ARG_CHECK( B->planned, "batch has no plan (apsm_batch_plan)" );
+    HIP_OR_THROW( hipEventRecord( B->ev0, B->stream ) );
+    if ( B->useGraph )
+        HIP_OR_THROW( hipGraphLaunch( B->exec, B->stream ) );
+    else
+        for ( auto& l : B->launches )
+            l();
+    HIP_OR_THROW( hipEventRecord( B->ev1, B->stream ) );
+    B->timed = true;
+}
+
+} // namespace

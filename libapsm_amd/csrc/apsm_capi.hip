@@ -8,10 +8,13 @@
 #include "apsm_device.h"
 #include "apsm_train2.h"
 #include "apsm_train_nc.h"
+#include "apsm_batch.h"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <functional>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -777,6 +780,8 @@ float elapsedMs( hipEvent_t a, hipEvent_t b )
 
 } // namespace
 
+#include "apsm_batch_host.h"
+
 extern "C" {
 
 void apsm_params_default( apsm_params* p )
@@ -1435,6 +1440,113 @@ int lls_synchronize( lls_handle* h )
     return guarded( [ & ] {
         ARG_CHECK( h != nullptr, "null handle" );
         HIP_OR_THROW( hipStreamSynchronize( h->stream ) );
+    } );
+}
+
+// ---- batched instances ----
+int apsm_batch_create( uint32_t n, const apsm_params* params, const uint32_t* num_antennas, const uint32_t* num_users, apsm_batch** out )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( out && params && num_antennas && num_users, "null argument" );
+        ARG_CHECK( n >= 1 && n <= 65535, "batch size must be in [1, 65535]" );
+        *out = nullptr;
+        apsm_batch* B = new apsm_batch();
+        try
+        {
+            HIP_OR_THROW( hipStreamCreateWithFlags( &B->stream, hipStreamNonBlocking ) );
+            HIP_OR_THROW( hipEventCreate( &B->ev0 ) );
+            HIP_OR_THROW( hipEventCreate( &B->ev1 ) );
+            int dev = 0;
+            HIP_OR_THROW( hipGetDevice( &dev ) );
+            HIP_OR_THROW( hipDeviceGetAttribute( &B->cus, hipDeviceAttributeMultiprocessorCount, dev ) );
+            for ( uint32_t i = 0; i < n; i++ )
+            {
+                apsm_handle* h = nullptr;
+                if ( apsm_create( &params[ i ], num_antennas[ i ], num_users[ i ], &h ) != APSM_OK )
+                    throw ApsmError { APSM_ERR_ARG, std::string( "instance " ) + std::to_string( i ) + ": " + g_lastError };
+                B->inst.push_back( h );
+            }
+        }
+        catch ( ... )
+        {
+            apsm_batch_destroy( B );
+            throw;
+        }
+        *out = B;
+    } );
+}
+
+void apsm_batch_destroy( apsm_batch* B )
+{
+    if ( !B )
+        return;
+    if ( B->stream )
+        (void)hipStreamSynchronize( B->stream );
+    if ( B->exec )
+        (void)hipGraphExecDestroy( B->exec );
+    if ( B->graph )
+        (void)hipGraphDestroy( B->graph );
+    for ( apsm_handle* h : B->inst )
+        apsm_destroy( h );
+    B->desc.release();
+    if ( B->ev0 )
+        (void)hipEventDestroy( B->ev0 );
+    if ( B->ev1 )
+        (void)hipEventDestroy( B->ev1 );
+    if ( B->stream )
+        (void)hipStreamDestroy( B->stream );
+    delete B;
+}
+
+int apsm_batch_plan( apsm_batch* B, const float* const* d_rx_train, const float* const* d_tx_train, const uint32_t* T,
+                     const float* const* d_rx_data, const uint32_t* M, float* const* d_est, int use_graph )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( B && d_rx_train && d_tx_train && T && d_rx_data && M && d_est, "null argument" );
+        B->planned = false;
+        batchPlan( B, d_rx_train, d_tx_train, T, d_rx_data, M, d_est, use_graph != 0 );
+    } );
+}
+
+int apsm_batch_run( apsm_batch* B )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( B != nullptr, "null batch" );
+        batchRun( B );
+    } );
+}
+
+int apsm_batch_synchronize( apsm_batch* B )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( B != nullptr, "null batch" );
+        HIP_OR_THROW( hipStreamSynchronize( B->stream ) );
+    } );
+}
+
+int apsm_batch_last_ms( apsm_batch* B, float* ms )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( B && ms, "null argument" );
+        ARG_CHECK( B->timed, "no run recorded" );
+        *ms = elapsedMs( B->ev0, B->ev1 );
+    } );
+}
+
+apsm_handle* apsm_batch_instance( apsm_batch* B, uint32_t i ) { return ( B && i < B->inst.size() ) ? B->inst[ i ] : nullptr; }
+
+int apsm_batch_tuning( apsm_batch* B, int32_t* table, uint32_t cap_rows, uint32_t* rows )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( B && rows, "null argument" );
+        *rows = (uint32_t)B->tuning.size();
+        if ( table )
+        {
+            ARG_CHECK( cap_rows >= B->tuning.size(), "tuning table too small" );
+            for ( size_t r = 0; r < B->tuning.size(); r++ )
+                for ( int c = 0; c < 5; c++ )
+                    table[ r * 5 + c ] = B->tuning[ r ][ c ];
+        }
     } );
 }
 

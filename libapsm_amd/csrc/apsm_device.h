@@ -62,11 +62,10 @@ __device__ __forceinline__ float waveSum( float v )
 // Staging: host layout rx [N][S][2] (+ optional permutation) -> stream rows Z [S][2N] and
 // norms; tx [U][S][2] -> Y [U][2S]; C of the new rows cleared.
 // ---------------------------------------------------------------------------------------
-__global__ void stage_pilots( const float* __restrict__ rx, const float* __restrict__ tx, const int32_t* __restrict__ perm,
-                              int N, int U, int64_t S, float* __restrict__ Z, float* __restrict__ Zn, float* __restrict__ Y,
-                              float* __restrict__ C, int64_t ldYC )
+__device__ __forceinline__ void stagePilotsBody( const float* __restrict__ rx, const float* __restrict__ tx, const int32_t* __restrict__ perm,
+                                                 int N, int U, int64_t S, float* __restrict__ Z, float* __restrict__ Zn,
+                                                 float* __restrict__ Y, float* __restrict__ C, int64_t ldYC, const int64_t k )
 {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if ( k >= S )
         return;
     const int64_t src = perm ? perm[ k ] : k;
@@ -88,6 +87,13 @@ __global__ void stage_pilots( const float* __restrict__ rx, const float* __restr
         C[ u * ldYC + 2 * k ] = 0.f;
         C[ u * ldYC + 2 * k + 1 ] = 0.f;
     }
+}
+
+__global__ void stage_pilots( const float* __restrict__ rx, const float* __restrict__ tx, const int32_t* __restrict__ perm, int N, int U,
+                              int64_t S, float* __restrict__ Z, float* __restrict__ Zn, float* __restrict__ Y, float* __restrict__ C,
+                              int64_t ldYC )
+{
+    stagePilotsBody( rx, tx, perm, N, U, S, Z, Zn, Y, C, ldYC, (int64_t)blockIdx.x * blockDim.x + threadIdx.x );
 }
 
 // ---------------------------------------------------------------------------------------
@@ -367,11 +373,10 @@ __global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq( const float* __
         thOld[ u * L + i ] = sTh[ i ];
 }
 
-// theta = thOld + sum over dictionary rows of c_j x_j (one workgroup per user).
-__global__ void apsm_theta_full( const float* __restrict__ Z, const float* __restrict__ C, const float* __restrict__ thOld,
-                                 float* __restrict__ theta, int N, int64_t j0, int64_t j1, int64_t ldC )
+// theta = thOld + sum over dictionary rows of c_j x_j (one workgroup of 256 threads per user u).
+__device__ __forceinline__ void thetaFullBody( const float* __restrict__ Z, const float* __restrict__ C, const float* __restrict__ thOld,
+                                               float* __restrict__ theta, int N, int64_t j0, int64_t j1, int64_t ldC, const int u )
 {
-    const int u = blockIdx.x;
     const int L = 2 * N;
     __shared__ float red[ 4 ][ 256 ];
     const float* Cu = C + u * ldC;
@@ -399,6 +404,12 @@ __global__ void apsm_theta_full( const float* __restrict__ Z, const float* __res
     }
 }
 
+__global__ void apsm_theta_full( const float* __restrict__ Z, const float* __restrict__ C, const float* __restrict__ thOld,
+                                 float* __restrict__ theta, int N, int64_t j0, int64_t j1, int64_t ldC )
+{
+    thetaFullBody( Z, C, thOld, theta, N, j0, j1, ldC, blockIdx.x );
+}
+
 // ---------------------------------------------------------------------------------------
 // Detect (cpp/lib/apsm/apsm_detect.cu:674-876 computes the same f per real row).
 // One thread per complex data sample, reading rx directly in the host layout [N][M][2];
@@ -407,12 +418,13 @@ __global__ void apsm_theta_full( const float* __restrict__ Z, const float* __res
 // est[u][m] = ( wL th_u.a_m + wG sum_j(c_2j E0 + c_2j+1 E1),
 //               wL th_u.b_m + wG sum_j(c_2j E2 + c_2j+1 E0) )
 // ---------------------------------------------------------------------------------------
-template <int NMAX, int UMAX>
-__global__ void __launch_bounds__( kDetThreads ) apsm_detect_kernel( const float* __restrict__ rx, int64_t M,
-                                                                    const float* __restrict__ Z, const float* __restrict__ Zn,
-                                                                    const float* __restrict__ C, int64_t ldC,
-                                                                    const float* __restrict__ theta, int64_t G, int U,
-                                                                    float wL, float wG, float g2, float* __restrict__ est, int N )
+// One tile of THREADS samples m = tile*THREADS + threadIdx.x (outputs only for m < M). Every
+// thread of the workgroup takes part in the tile loads and barriers.
+template <int NMAX, int UMAX, int THREADS>
+__device__ __forceinline__ void detectTile( const float* __restrict__ rx, int64_t M, const float* __restrict__ Z,
+                                            const float* __restrict__ Zn, const float* __restrict__ C, int64_t ldC,
+                                            const float* __restrict__ theta, int64_t G, int U, float wL, float wG, float g2,
+                                            float* __restrict__ est, int N, const int64_t tile )
 {
     // tiles are stored with the antenna count padded to NMAX (zeros), so the inner loops
     // carry no bounds checks for N < NMAX
@@ -422,7 +434,7 @@ __global__ void __launch_bounds__( kDetThreads ) apsm_detect_kernel( const float
     __shared__ float sC[ UMAX * 2 * kDetTile ];
     const int L = 2 * N;
 
-    const int64_t m = (int64_t)blockIdx.x * kDetThreads + threadIdx.x;
+    const int64_t m = tile * THREADS + threadIdx.x;
     const bool valid = m < M;
     float xr[ NMAX ], xi[ NMAX ];
     float xn = 0.f;
@@ -443,16 +455,16 @@ __global__ void __launch_bounds__( kDetThreads ) apsm_detect_kernel( const float
     {
         const int nt = ( G - j0 ) < kDetTile ? (int)( G - j0 ) : kDetTile;
         __syncthreads();
-        for ( int i = threadIdx.x; i < kDetTile * NMAX; i += kDetThreads )
+        for ( int i = threadIdx.x; i < kDetTile * NMAX; i += THREADS )
         {
             const int j = i / NMAX, n = i % NMAX;
             const bool ok = j < nt && n < N;
             sZr[ i ] = ok ? Z[ ( j0 + j ) * L + n ] : 0.f;
             sZi[ i ] = ok ? Z[ ( j0 + j ) * L + N + n ] : 0.f;
         }
-        for ( int i = threadIdx.x; i < kDetTile; i += kDetThreads )
+        for ( int i = threadIdx.x; i < kDetTile; i += THREADS )
             sZn[ i ] = i < nt ? Zn[ j0 + i ] : 0.f;
-        for ( int i = threadIdx.x; i < UMAX * 2 * kDetTile; i += kDetThreads )
+        for ( int i = threadIdx.x; i < UMAX * 2 * kDetTile; i += THREADS )
         {
             int uu = i / ( 2 * kDetTile ), jj = i % ( 2 * kDetTile );
             sC[ i ] = ( uu < U && jj < 2 * nt ) ? C[ uu * ldC + 2 * j0 + jj ] : 0.f;
@@ -484,28 +496,39 @@ __global__ void __launch_bounds__( kDetThreads ) apsm_detect_kernel( const float
             }
         }
     }
-    if ( !valid )
-        return;
-#pragma unroll
-    for ( int uu = 0; uu < UMAX; uu++ )
+    if ( valid )
     {
-        if ( uu < U )
-        {
-            const float* th = theta + uu * L;
-            float la = 0.f, lb = 0.f;
 #pragma unroll
-            for ( int n = 0; n < NMAX; n++ )
+        for ( int uu = 0; uu < UMAX; uu++ )
+        {
+            if ( uu < U )
             {
-                if ( n < N )
+                const float* th = theta + uu * L;
+                float la = 0.f, lb = 0.f;
+#pragma unroll
+                for ( int n = 0; n < NMAX; n++ )
                 {
-                    la += th[ n ] * xr[ n ] + th[ N + n ] * xi[ n ];
-                    lb += th[ n ] * xi[ n ] - th[ N + n ] * xr[ n ];
+                    if ( n < N )
+                    {
+                        la += th[ n ] * xr[ n ] + th[ N + n ] * xi[ n ];
+                        lb += th[ n ] * xi[ n ] - th[ N + n ] * xr[ n ];
+                    }
                 }
+                float2 o = make_float2( wL * la + wG * fa[ uu ], wL * lb + wG * fb[ uu ] );
+                reinterpret_cast<float2*>( est )[ (int64_t)uu * M + m ] = o;
             }
-            float2 o = make_float2( wL * la + wG * fa[ uu ], wL * lb + wG * fb[ uu ] );
-            reinterpret_cast<float2*>( est )[ (int64_t)uu * M + m ] = o;
         }
     }
+}
+
+template <int NMAX, int UMAX>
+__global__ void __launch_bounds__( kDetThreads ) apsm_detect_kernel( const float* __restrict__ rx, int64_t M,
+                                                                    const float* __restrict__ Z, const float* __restrict__ Zn,
+                                                                    const float* __restrict__ C, int64_t ldC,
+                                                                    const float* __restrict__ theta, int64_t G, int U,
+                                                                    float wL, float wG, float g2, float* __restrict__ est, int N )
+{
+    detectTile<NMAX, UMAX, kDetThreads>( rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, blockIdx.x );
 }
 
 // ---------------------------------------------------------------------------------------

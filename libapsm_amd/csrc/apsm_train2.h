@@ -63,15 +63,13 @@ struct EntryArgs
     float wL, wG, g2;
 };
 
-// One workgroup per entering complex sample.
+// One workgroup per entering complex sample i.
 template <int UMAX>
-__global__ void __launch_bounds__( kEntryThreads ) apsm_entry_frozen( const float* __restrict__ Z, const float* __restrict__ Zn,
-                                                                     const float* __restrict__ C, const float* __restrict__ thF,
-                                                                     float* __restrict__ P, const EntryArgs a )
+__device__ __forceinline__ void entryFrozenBody( const float* __restrict__ Z, const float* __restrict__ Zn, const float* __restrict__ C,
+                                                 const float* __restrict__ thF, float* __restrict__ P, const EntryArgs& a, const int i )
 {
     __shared__ float sx[ 128 ];
     __shared__ float sred[ kEntryThreads / kWave ][ 2 * UMAX ];
-    const int i = blockIdx.x;
     const int tid = threadIdx.x;
     const int N = a.N, L = a.L;
     const int64_t tc = a.newC0 + i;
@@ -136,6 +134,14 @@ __global__ void __launch_bounds__( kEntryThreads ) apsm_entry_frozen( const floa
     }
 }
 
+template <int UMAX>
+__global__ void __launch_bounds__( kEntryThreads ) apsm_entry_frozen( const float* __restrict__ Z, const float* __restrict__ Zn,
+                                                                     const float* __restrict__ C, const float* __restrict__ thF,
+                                                                     float* __restrict__ P, const EntryArgs a )
+{
+    entryFrozenBody<UMAX>( Z, Zn, C, thF, P, a, blockIdx.x );
+}
+
 struct GramArgs
 {
     int N, L;
@@ -144,10 +150,10 @@ struct GramArgs
     float wL, wG, g2;
 };
 
-// Gact[tri(i) + j] (3 floats) for active samples i >= j (relative to c0).
-__global__ void apsm_active_gram( const float* __restrict__ Z, const float* __restrict__ Zn, float* __restrict__ Gact, const GramArgs a )
+// Gact[tri(i) + j] (3 floats) for active samples i >= j (relative to c0); one thread per pair idx.
+__device__ __forceinline__ void activeGramBody( const float* __restrict__ Z, const float* __restrict__ Zn, float* __restrict__ Gact,
+                                                const GramArgs& a, const int64_t idx )
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = triIdx( a.nA );
     if ( idx >= total )
         return;
@@ -169,6 +175,11 @@ __global__ void apsm_active_gram( const float* __restrict__ Z, const float* __re
     Gact[ idx * 3 + 2 ] = -a.wL * sp + a.wG * E2;
 }
 
+__global__ void apsm_active_gram( const float* __restrict__ Z, const float* __restrict__ Zn, float* __restrict__ Gact, const GramArgs a )
+{
+    activeGramBody( Z, Zn, Gact, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x );
+}
+
 struct EvArgs
 {
     int N, L, U, q, w2, nSteps;
@@ -180,11 +191,10 @@ struct EvArgs
 // the window row at position 2*pc + part (row s_k - W_k + 2*pc + part) if it entered before
 // step k; zero otherwise. One thread per (step, complex window position); the kernel values
 // of the (at most kNewMax) leaving samples are computed once and contracted for every user.
-__global__ void apsm_evict_corr( const float* __restrict__ Z, const float* __restrict__ Zn, const float* __restrict__ C,
-                                 float* __restrict__ EvF, const EvArgs a )
+__device__ __forceinline__ void evictCorrBody( const float* __restrict__ Z, const float* __restrict__ Zn, const float* __restrict__ C,
+                                               float* __restrict__ EvF, const EvArgs& a, const int64_t idx )
 {
     const int Wc = a.w2 / 2;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if ( idx >= (int64_t)a.nSteps * Wc )
         return;
     const int kk = (int)( idx / Wc ), pc = (int)( idx % Wc );
@@ -232,6 +242,12 @@ __global__ void apsm_evict_corr( const float* __restrict__ Z, const float* __res
         o[ 2 * pc ] = a.wG * dFa;
         o[ 2 * pc + 1 ] = a.wG * dFb;
     }
+}
+
+__global__ void apsm_evict_corr( const float* __restrict__ Z, const float* __restrict__ Zn, const float* __restrict__ C,
+                                 float* __restrict__ EvF, const EvArgs a )
+{
+    evictCorrBody( Z, Zn, C, EvF, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x );
 }
 
 struct Seq2Args
@@ -293,17 +309,15 @@ __device__ __forceinline__ void waitVmN( int n )
 //   [+0, +2*NEWPER)          their frozen shares P
 //   [+0, +w2)                eviction shares of the window rows
 template <int TP, int NEWPER, int PMAX>
-__global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* __restrict__ Z, const float* __restrict__ Zn,
-                                                                const float* __restrict__ Y, float* __restrict__ C,
-                                                                float* __restrict__ Fg, float* __restrict__ thF,
-                                                                const float* __restrict__ P, const float* __restrict__ Gact,
-                                                                const float* __restrict__ EvF, const Seq2Args a )
+__device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const float* __restrict__ Zn, const float* __restrict__ Y,
+                                          float* __restrict__ C, float* __restrict__ Fg, float* __restrict__ thF,
+                                          const float* __restrict__ P, const float* __restrict__ Gact, const float* __restrict__ EvF,
+                                          const Seq2Args& a, const int u )
 {
     constexpr int kDpw = ( NEWPER * 3 * kActMax + 2 * NEWPER + kSeqThreads + kSeqThreads - 1 ) / kSeqThreads; // DMAs per wave per step
     constexpr int kPkCap = kDpw * kSeqThreads; // floats per package buffer
     constexpr int TPA = ( TP + 3 ) & ~3; // column segment per part, padded for float4 reads
     constexpr int kColStride = kMaxParts * TPA; // floats per column array (A, B or C)
-    const int u = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & ( kWave - 1 ), wave = tid / kWave;
     const int Wc = a.Wc, w2 = a.w2, L = a.L;
     const int v = tid % Wc, part = tid / Wc;
@@ -663,6 +677,16 @@ __global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* _
             acc += sRedF[ ch * L + tid ];
         thF[ u * L + tid ] = acc;
     }
+}
+
+template <int TP, int NEWPER, int PMAX>
+__global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* __restrict__ Z, const float* __restrict__ Zn,
+                                                                const float* __restrict__ Y, float* __restrict__ C,
+                                                                float* __restrict__ Fg, float* __restrict__ thF,
+                                                                const float* __restrict__ P, const float* __restrict__ Gact,
+                                                                const float* __restrict__ EvF, const Seq2Args a )
+{
+    seq2Body<TP, NEWPER, PMAX>( Z, Zn, Y, C, Fg, thF, P, Gact, EvF, a, blockIdx.x );
 }
 
 // LDS bytes of apsm_train_seq2<TP, NEWPER> for a window of w2 real rows.
