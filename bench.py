@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-lls", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-instances measurement")
+    ap.add_argument("--batched-instances", type=int, default=512)
     return ap.parse_args()
 
 
@@ -102,6 +104,90 @@ def cpu_baseline(w, data, budget_s):
                    f"({t_train:.2f} s) and {Ms} samples detected ({t_det:.2f} s); per-kernel-evaluation "
                    f"times scaled to {U} users x ({w['T']} pilots, {w['M']} samples): "
                    f"train {U * full_train:.0f} s + detect {U * full_det:.0f} s"),
+    }
+
+
+def bench_batched(count, steps):
+    """Batched workload: `count` independent instances (antennas in {4, 8, 16, 32}, QPSK or
+    16QAM, dictionary 200..4000, windowSize 40/80/160, 10k..100k detect samples, pilots =
+    dictionary size) run as one planned, graph-captured batch; the same instances through the
+    plain single-handle path one after another give the comparison. Value: detected symbols/s
+    of the whole batch (reset + train + detect of every instance per run)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from libapsm_amd import ApsmBatch, ApsmNomaDetectorFactory, datagen, lib
+    from libapsm_amd._lib import DeviceArray, check
+
+    inst = datagen.batched_instances(count)
+
+    def gen(x):
+        return datagen.generate(x["N"], x["U_tx"], x["T"], x["M"], modulation=x["modulation"], snr_db=x["snr_db"], seed=x["seed"])
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        data = list(ex.map(gen, inst))
+    t_gen = time.perf_counter() - t0
+    cfgs = [{"windowSize": x["window"], "dictionarySize": x["dict_size"], "eB": x["eB"],
+             "gaussianKernelVariance": datagen.kernel_variance_for_sigma(x["sigma"], x["N"])} for x in inst]
+    up = lambda z: DeviceArray(np.ascontiguousarray(z, dtype=np.complex64).view(np.float32))  # noqa: E731
+    d_rxt = [up(d["rx_train"]) for d in data]
+    d_txt = [up(d["tx_train"][x["user"]][None, :]) for d, x in zip(data, inst)]
+    d_rxd = [up(d["rx_data"]) for d in data]
+    d_est = [DeviceArray.empty((1, x["M"], 2), np.float32) for x in inst]
+    T = [x["T"] for x in inst]
+    M = [x["M"] for x in inst]
+    symbols = float(sum(M))
+
+    b = ApsmBatch(cfgs, [x["N"] for x in inst])
+    t0 = time.perf_counter()
+    b.plan_device([d.ptr for d in d_rxt], [d.ptr for d in d_txt], T, [d.ptr for d in d_rxd], M, [d.ptr for d in d_est])
+    b.synchronize()
+    t_plan = time.perf_counter() - t0
+    b.run()
+    b.synchronize()
+    t0 = time.perf_counter()
+    dev_ms, split = [], []
+    for _ in range(steps):
+        b.run()
+        dev_ms.append(b.last_ms())
+        split.append(b.last_split())
+    b.synchronize()
+    batch_s = (time.perf_counter() - t0) / steps
+    est_b = [d.numpy() for d in d_est]
+
+    # plain path: one handle per instance, one instance after another
+    dets = [ApsmNomaDetectorFactory.build(dict(c, otype="APSM"), x["N"]) for c, x in zip(cfgs, inst)]
+    check(lib().apsm_device_synchronize())
+    t0 = time.perf_counter()
+    for i, det in enumerate(dets):
+        det.resetState()
+        det.train_device(d_rxt[i].ptr, d_txt[i].ptr, T[i])
+        det.detect_device(d_rxd[i].ptr, M[i], d_est[i].ptr)
+        det.synchronize()
+    plain_s = time.perf_counter() - t0
+    same = all(np.array_equal(est_b[i], d_est[i].numpy()) for i in range(count))
+    errs = 0
+    for i, (d, x) in enumerate(zip(data, inst)):
+        e = est_b[i][0]
+        errs += datagen.symbol_errors(e[:, 0] + 1j * e[:, 1], d["tx_data"][x["user"]], d["bps"])
+    return {
+        "workload": f"batched-{count}",
+        "instances": count,
+        "detected_symbols": int(symbols),
+        "value": symbols / batch_s,
+        "unit": "detected symbols/s",
+        "ms_per_run": 1000 * batch_s,
+        "device_ms_per_run": float(np.mean(dev_ms)),
+        "device_train_ms": float(np.mean([t for t, _ in split])),
+        "device_detect_ms": float(np.mean([d for _, d in split])),
+        "plain_value": symbols / plain_s,
+        "plain_ms": 1000 * plain_s,
+        "speedup_vs_plain": plain_s / batch_s,
+        "identical_to_plain": bool(same),
+        "ser": errs / symbols,
+        "plan_s": t_plan,
+        "datagen_s": t_gen,
+        "tuning": b.tuning(),
     }
 
 
@@ -208,6 +294,9 @@ def main():
                       "apsm_ser_same_user": datagen.symbol_errors(est_c[0], tx_data[0], data["bps"]) / M}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, data, args.cpu_seconds)
+    if not args.no_batched:
+        del d_rxt, d_txt, d_rxd, d_est, det
+        out["batched"] = bench_batched(args.batched_instances, max(1, args.steps))
     print(json.dumps(out))
 
 

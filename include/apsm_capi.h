@@ -203,6 +203,8 @@ int apsm_batch_run(apsm_batch* b);
 int apsm_batch_synchronize(apsm_batch* b);
 /* Device time of the most recent run (HIP events around it on the batch stream), in ms. */
 int apsm_batch_last_ms(apsm_batch* b, float* ms);
+/* The same run split at the end of the train launches: train and detect device time, in ms. */
+int apsm_batch_last_split(apsm_batch* b, float* train_ms, float* detect_ms);
 /* Instance i's handle (state readback with apsm_get_state; owned by the batch). */
 apsm_handle* apsm_batch_instance(apsm_batch* b, uint32_t i);
 /* Autotuner choices: rows of (NMAX, UMAX, threads per workgroup, workgroups per CU, work items). */

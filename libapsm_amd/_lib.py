@@ -87,6 +87,7 @@ SIGNATURES = {
     "apsm_batch_run": (ctypes.c_int, [_vp]),
     "apsm_batch_synchronize": (ctypes.c_int, [_vp]),
     "apsm_batch_last_ms": (ctypes.c_int, [_vp, _f32p]),
+    "apsm_batch_last_split": (ctypes.c_int, [_vp, _f32p, _f32p]),
     "apsm_batch_instance": (_vp, [_vp, _u32]),
     "apsm_batch_tuning": (ctypes.c_int, [_vp, _P(ctypes.c_int32), _u32, _P(_u32)]),
     "apsm_stream_create": (ctypes.c_int, [_P(_vp)]),

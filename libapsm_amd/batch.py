@@ -68,6 +68,12 @@ class ApsmBatch:
         check(lib().apsm_batch_last_ms(self._b, ctypes.byref(ms)))
         return ms.value
 
+    def last_split(self):
+        """(train ms, detect ms) of the most recent run."""
+        t, d = ctypes.c_float(), ctypes.c_float()
+        check(lib().apsm_batch_last_split(self._b, ctypes.byref(t), ctypes.byref(d)))
+        return t.value, d.value
+
     def estimates(self, i):
         """complex64 [U_i][M_i] of the last run (after synchronize)."""
         e = self.d_est[i].numpy()

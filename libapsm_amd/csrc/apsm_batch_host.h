@@ -11,16 +11,21 @@
 #include <map>
 #include <tuple>
 
+constexpr int kBatchSideStreams = 3; // sequencer buckets of one round run side by side
+
 struct apsm_batch
 {
     std::vector<apsm_handle*> inst;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t side[ kBatchSideStreams ] = {};
+    hipEvent_t fork = nullptr, join[ kBatchSideStreams ] = {};
+    hipEvent_t ev0 = nullptr, evMid = nullptr, ev1 = nullptr;
     bool planned = false, useGraph = true, timed = false;
     DevBuf<char> desc; // every descriptor array of the plan
-    std::vector<std::function<void()>> launches; // the run, in order (kernel launches only)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    std::vector<std::function<void()>> launches; // the run, in order (train launches, then detect)
+    size_t trainLaunches = 0;
+    hipGraph_t graph[ 2 ] = {};
+    hipGraphExec_t exec[ 2 ] = {}; // train, detect
     std::vector<std::array<int, 5>> tuning; // (NMAX, UMAX, threads, blocks per CU, items)
     int cus = 256;
 };
@@ -164,15 +169,14 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
                               "batched instances need the blocked train path, a configured kernel variance, no LLS initialisation, "
                               "no shuffle and at most 16 users" };
     }
-    if ( B->exec )
+    for ( int g = 0; g < 2; g++ )
     {
-        (void)hipGraphExecDestroy( B->exec );
-        B->exec = nullptr;
-    }
-    if ( B->graph )
-    {
-        (void)hipGraphDestroy( B->graph );
-        B->graph = nullptr;
+        if ( B->exec[ g ] )
+            (void)hipGraphExecDestroy( B->exec[ g ] );
+        if ( B->graph[ g ] )
+            (void)hipGraphDestroy( B->graph[ g ] );
+        B->exec[ g ] = nullptr;
+        B->graph[ g ] = nullptr;
     }
     B->launches.clear();
     B->tuning.clear();
@@ -448,20 +452,39 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
                                  HIP_OR_THROW( hipGetLastError() );
                              } } );
         }
-        for ( auto& kv : seq )
+        // the round's sequencer buckets are independent: fork them onto side streams
+        struct SeqLaunch
         {
-            const size_t off = arena.add( kv.second );
-            const int cnt = (int)kv.second.size(), tp = kv.first.first, newPer = kv.first.second;
-            const int gx = seqDim[ kv.first ].first;
-            const size_t lds = seqDim[ kv.first ].second;
-            pend.push_back( { 0, [ = ]( const char* base ) {
-                                 const apsm::Seq2Desc* d = (const apsm::Seq2Desc*)( base + off );
-                                 if ( newPer == 1 )
-                                     launchBatchSeq2Tp<1>( B->stream, tp, d, cnt, gx, lds );
+            size_t off;
+            int cnt, tp, newPer, gx;
+            size_t lds;
+        };
+        std::vector<SeqLaunch> sl;
+        for ( auto& kv : seq )
+            sl.push_back( SeqLaunch { arena.add( kv.second ), (int)kv.second.size(), kv.first.first, kv.first.second,
+                                      seqDim[ kv.first ].first, seqDim[ kv.first ].second } );
+        pend.push_back( { 0, [ = ]( const char* base ) {
+                             const int nside = std::min<int>( kBatchSideStreams, (int)sl.size() - 1 );
+                             if ( nside > 0 )
+                                 HIP_OR_THROW( hipEventRecord( B->fork, B->stream ) );
+                             for ( int k = 0; k < nside; k++ )
+                                 HIP_OR_THROW( hipStreamWaitEvent( B->side[ k ], B->fork, 0 ) );
+                             for ( size_t k = 0; k < sl.size(); k++ )
+                             {
+                                 const int lane = (int)( k % ( nside + 1 ) );
+                                 hipStream_t st = lane == 0 ? B->stream : B->side[ lane - 1 ];
+                                 const apsm::Seq2Desc* d = (const apsm::Seq2Desc*)( base + sl[ k ].off );
+                                 if ( sl[ k ].newPer == 1 )
+                                     launchBatchSeq2Tp<1>( st, sl[ k ].tp, d, sl[ k ].cnt, sl[ k ].gx, sl[ k ].lds );
                                  else
-                                     launchBatchSeq2Tp<2>( B->stream, tp, d, cnt, gx, lds );
-                             } } );
-        }
+                                     launchBatchSeq2Tp<2>( st, sl[ k ].tp, d, sl[ k ].cnt, sl[ k ].gx, sl[ k ].lds );
+                             }
+                             for ( int k = 0; k < nside; k++ )
+                             {
+                                 HIP_OR_THROW( hipEventRecord( B->join[ k ], B->side[ k ] ) );
+                                 HIP_OR_THROW( hipStreamWaitEvent( B->stream, B->join[ k ], 0 ) );
+                             }
+                         } } );
     }
     {
         int maxU = 1;
@@ -563,24 +586,30 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
     (void)hipEventDestroy( t1 );
 
     B->useGraph = useGraph;
+    B->trainLaunches = trainLaunches;
     if ( useGraph )
     {
-        HIP_OR_THROW( hipStreamBeginCapture( B->stream, hipStreamCaptureModeRelaxed ) );
-        try
+        // two graphs (train, detect) so a run can time its two parts
+        for ( int g = 0; g < 2; g++ )
         {
-            for ( auto& l : B->launches )
-                l();
+            const size_t l0 = g == 0 ? 0 : trainLaunches, l1 = g == 0 ? trainLaunches : B->launches.size();
+            HIP_OR_THROW( hipStreamBeginCapture( B->stream, hipStreamCaptureModeRelaxed ) );
+            try
+            {
+                for ( size_t l = l0; l < l1; l++ )
+                    B->launches[ l ]();
+            }
+            catch ( ... )
+            {
+                hipGraph_t gg = nullptr;
+                (void)hipStreamEndCapture( B->stream, &gg );
+                if ( gg )
+                    (void)hipGraphDestroy( gg );
+                throw;
+            }
+            HIP_OR_THROW( hipStreamEndCapture( B->stream, &B->graph[ g ] ) );
+            HIP_OR_THROW( hipGraphInstantiate( &B->exec[ g ], B->graph[ g ], nullptr, nullptr, 0 ) );
         }
-        catch ( ... )
-        {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture( B->stream, &g );
-            if ( g )
-                (void)hipGraphDestroy( g );
-            throw;
-        }
-        HIP_OR_THROW( hipStreamEndCapture( B->stream, &B->graph ) );
-        HIP_OR_THROW( hipGraphInstantiate( &B->exec, B->graph, nullptr, nullptr, 0 ) );
     }
     B->planned = true;
 }
@@ -590,10 +619,20 @@ void batchRun( apsm_batch* B )
     ARG_CHECK( B->planned, "batch has no plan (apsm_batch_plan)" );
     HIP_OR_THROW( hipEventRecord( B->ev0, B->stream ) );
     if ( B->useGraph )
-        HIP_OR_THROW( hipGraphLaunch( B->exec, B->stream ) );
+    {
+        HIP_OR_THROW( hipGraphLaunch( B->exec[ 0 ], B->stream ) );
+        HIP_OR_THROW( hipEventRecord( B->evMid, B->stream ) );
+        HIP_OR_THROW( hipGraphLaunch( B->exec[ 1 ], B->stream ) );
+    }
     else
-        for ( auto& l : B->launches )
-            l();
+    {
+        for ( size_t l = 0; l < B->launches.size(); l++ )
+        {
+            if ( l == B->trainLaunches )
+                HIP_OR_THROW( hipEventRecord( B->evMid, B->stream ) );
+            B->launches[ l ]();
+        }
+    }
     HIP_OR_THROW( hipEventRecord( B->ev1, B->stream ) );
     B->timed = true;
 }

@@ -1454,7 +1454,14 @@ int apsm_batch_create( uint32_t n, const apsm_params* params, const uint32_t* nu
         try
         {
             HIP_OR_THROW( hipStreamCreateWithFlags( &B->stream, hipStreamNonBlocking ) );
+            for ( int k = 0; k < kBatchSideStreams; k++ )
+            {
+                HIP_OR_THROW( hipStreamCreateWithFlags( &B->side[ k ], hipStreamNonBlocking ) );
+                HIP_OR_THROW( hipEventCreateWithFlags( &B->join[ k ], hipEventDisableTiming ) );
+            }
+            HIP_OR_THROW( hipEventCreateWithFlags( &B->fork, hipEventDisableTiming ) );
             HIP_OR_THROW( hipEventCreate( &B->ev0 ) );
+            HIP_OR_THROW( hipEventCreate( &B->evMid ) );
             HIP_OR_THROW( hipEventCreate( &B->ev1 ) );
             int dev = 0;
             HIP_OR_THROW( hipGetDevice( &dev ) );
@@ -1482,17 +1489,29 @@ void apsm_batch_destroy( apsm_batch* B )
         return;
     if ( B->stream )
         (void)hipStreamSynchronize( B->stream );
-    if ( B->exec )
-        (void)hipGraphExecDestroy( B->exec );
-    if ( B->graph )
-        (void)hipGraphDestroy( B->graph );
+    for ( int k = 0; k < kBatchSideStreams; k++ )
+        if ( B->side[ k ] )
+            (void)hipStreamSynchronize( B->side[ k ] );
+    for ( int g = 0; g < 2; g++ )
+    {
+        if ( B->exec[ g ] )
+            (void)hipGraphExecDestroy( B->exec[ g ] );
+        if ( B->graph[ g ] )
+            (void)hipGraphDestroy( B->graph[ g ] );
+    }
     for ( apsm_handle* h : B->inst )
         apsm_destroy( h );
     B->desc.release();
-    if ( B->ev0 )
-        (void)hipEventDestroy( B->ev0 );
-    if ( B->ev1 )
-        (void)hipEventDestroy( B->ev1 );
+    for ( hipEvent_t e : { B->ev0, B->evMid, B->ev1, B->fork } )
+        if ( e )
+            (void)hipEventDestroy( e );
+    for ( int k = 0; k < kBatchSideStreams; k++ )
+    {
+        if ( B->join[ k ] )
+            (void)hipEventDestroy( B->join[ k ] );
+        if ( B->side[ k ] )
+            (void)hipStreamDestroy( B->side[ k ] );
+    }
     if ( B->stream )
         (void)hipStreamDestroy( B->stream );
     delete B;
@@ -1530,6 +1549,16 @@ int apsm_batch_last_ms( apsm_batch* B, float* ms )
         ARG_CHECK( B && ms, "null argument" );
         ARG_CHECK( B->timed, "no run recorded" );
         *ms = elapsedMs( B->ev0, B->ev1 );
+    } );
+}
+
+int apsm_batch_last_split( apsm_batch* B, float* train_ms, float* detect_ms )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( B && train_ms && detect_ms, "null argument" );
+        ARG_CHECK( B->timed, "no run recorded" );
+        *train_ms = elapsedMs( B->ev0, B->evMid );
+        *detect_ms = elapsedMs( B->evMid, B->ev1 );
     } );
 }
 
