@@ -220,7 +220,8 @@ public:
     {
         static const char* const validKeys[] = { "otype", "llsInitialization", "trainingShuffle", "gaussianKernelWeight",
                                                  "gaussianKernelVariance", "windowSize", "sampleStep", "trainPasses",
-                                                 "startWithFullWindow", "dictionarySize", "normConstraint", "eB", "detectVersion" };
+                                                 "startWithFullWindow", "dictionarySize", "normConstraint", "eB", "detectVersion",
+                                                 "detectPrecision" };
         for ( auto& item : config.items() )
         {
             bool ok = false;
@@ -242,7 +243,25 @@ public:
         par.dictionarySize = config.value( "dictionarySize", 685 );
         par.normConstraint = config.value( "normConstraint", 0. );
         par.eB = config.value( "eB", 0.001 );
+        par.detectPrecision = precision( config );
         return par;
+    }
+
+    // "detectPrecision": arithmetic of the detect cross terms (no counterpart in the reference)
+    static int32_t precision( const nlohmann::json& config )
+    {
+        const std::string v = config.value( "detectPrecision", "fp32" );
+        if ( v == "fp32" )
+            return APSM_PRECISION_FP32;
+        if ( v == "fp32-vector" )
+            return APSM_PRECISION_FP32_VECTOR;
+        if ( v == "fp32-matrix" )
+            return APSM_PRECISION_FP32_MATRIX;
+        if ( v == "bf16" )
+            return APSM_PRECISION_BF16;
+        if ( v == "fp16" )
+            return APSM_PRECISION_FP16;
+        throw std::invalid_argument( "Unknown detect precision: " + v );
     }
 
     static int32_t version( const nlohmann::json& config )

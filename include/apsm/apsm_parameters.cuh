@@ -27,6 +27,9 @@ public:
     RealSample normConstraint; ///< norm constraint for dictionary sparsification (0 disables)
     RealSample eB; ///< hyperslab width
 
+    // detect arithmetic (extension: APSM_PRECISION_* of the C ABI, key "detectPrecision")
+    int32_t detectPrecision = APSM_PRECISION_FP32;
+
     // The C ABI parameter block for a given detectVersion.
     apsm_params toC( int32_t detectVersion ) const
     {
@@ -45,6 +48,7 @@ public:
         p.lls_initialization = llsInitialization ? 1 : 0;
         p.training_shuffle = trainingShuffle ? 1 : 0;
         p.detect_version = detectVersion;
+        p.detect_precision = detectPrecision;
         return p;
     }
 };

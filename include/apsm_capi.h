@@ -40,6 +40,16 @@ extern "C" {
 #define APSM_VERSION_DETECT_SHMEM 0x10006
 #define APSM_VERSION_DETECT_BALANCED 0x10007
 
+/* detect_precision values: arithmetic of the detect cross terms x.z (no counterpart in the
+ * reference, whose detect is FP32 on the CUDA cores). Squared norms, distances, exponentials,
+ * the coefficient contraction and the linear term are FP32 in every mode; the 16-bit modes
+ * round only the cross-term inputs and accumulate in FP32. */
+#define APSM_PRECISION_FP32 0 /* FP32; the library picks the vector or matrix-core kernel by antenna count */
+#define APSM_PRECISION_FP32_VECTOR 1 /* FP32 on the vector ALUs */
+#define APSM_PRECISION_FP32_MATRIX 2 /* FP32 on the matrix cores */
+#define APSM_PRECISION_BF16 3 /* BF16 inputs on the matrix cores, FP32 accumulation */
+#define APSM_PRECISION_FP16 4 /* FP16 inputs on the matrix cores, FP32 accumulation */
+
 /* Parameter set. Field meaning, units and defaults follow apsm_parameters
  * (cpp/lib/apsm/apsm_parameters.cuh:68-86) and the factory defaults
  * (cpp/lib/apsm/apsm_noma_detector.cu:361-372). window_size and dictionary_size count
@@ -63,6 +73,7 @@ typedef struct apsm_params
     uint8_t reserved;
     int32_t detect_version;
     float linear_kernel_weight; /* w_L; the reference factory sets 1 - gaussian_kernel_weight */
+    int32_t detect_precision; /* APSM_PRECISION_* (default APSM_PRECISION_FP32) */
 } apsm_params;
 
 typedef struct apsm_handle apsm_handle;
@@ -207,7 +218,8 @@ int apsm_batch_last_ms(apsm_batch* b, float* ms);
 int apsm_batch_last_split(apsm_batch* b, float* train_ms, float* detect_ms);
 /* Instance i's handle (state readback with apsm_get_state; owned by the batch). */
 apsm_handle* apsm_batch_instance(apsm_batch* b, uint32_t i);
-/* Autotuner choices: rows of (NMAX, UMAX, threads per workgroup, workgroups per CU, work items). */
+/* Autotuner choices: rows of (NMAX, UMAX, threads per workgroup, workgroups per CU, work items,
+ * kernel: -1 vector ALU, else the matrix-core input precision 0 FP32, 1 BF16, 2 FP16). */
 int apsm_batch_tuning(apsm_batch* b, int32_t* table, uint32_t cap_rows, uint32_t* rows);
 
 /* CudaStream (cpp/lib/cuda/cuda_stream.cuh:67-96): a HIP stream of default priority. */

@@ -34,6 +34,7 @@ class ApsmParamsC(ctypes.Structure):
         ("reserved", ctypes.c_uint8),
         ("detect_version", ctypes.c_int32),
         ("linear_kernel_weight", ctypes.c_float),
+        ("detect_precision", ctypes.c_int32),
     ]
 
 

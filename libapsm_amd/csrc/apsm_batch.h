@@ -9,6 +9,7 @@
 
 #include "apsm_device.h"
 #include "apsm_train2.h"
+#include "apsm_detect_launch.h"
 
 namespace apsm {
 
@@ -55,22 +56,6 @@ struct ThetaDesc
     float* theta;
     int64_t j0, j1, ldC;
     int N, U;
-};
-
-struct DetDesc
-{
-    const float* rx;
-    const float *Z, *Zn, *C, *theta;
-    float* est;
-    int64_t M, ldC, G;
-    int U, N;
-    float wL, wG, g2;
-};
-
-struct WorkItem
-{
-    int desc;
-    int tile;
 };
 
 __global__ void batch_stage( const StageDesc* __restrict__ d )

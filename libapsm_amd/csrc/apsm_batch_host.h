@@ -26,7 +26,7 @@ struct apsm_batch
     size_t trainLaunches = 0;
     hipGraph_t graph[ 2 ] = {};
     hipGraphExec_t exec[ 2 ] = {}; // train, detect
-    std::vector<std::array<int, 5>> tuning; // (NMAX, UMAX, threads, blocks per CU, items)
+    std::vector<std::array<int, 6>> tuning; // (NMAX, UMAX, threads, blocks per CU, items, kernel)
     int cus = 256;
 };
 
@@ -135,9 +135,6 @@ int pmaxOfTp( int tp )
     default: return 6;
     }
 }
-
-int roundUpPow2Users( int U ) { return U <= 1 ? 1 : U <= 2 ? 2 : U <= 4 ? 4 : U <= 8 ? 8 : 16; }
-int nmaxOf( int N ) { return N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64; }
 
 // Host-side descriptor storage: typed arrays appended into one byte buffer; launches refer to
 // offsets that become device pointers once the buffer is uploaded.
@@ -499,14 +496,19 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
     }
     const size_t trainLaunches = pend.size();
 
-    // detect buckets by (NMAX, UMAX); work items for every candidate tile size
-    std::map<std::pair<int, int>, std::vector<int>> dBuckets;
+    // detect buckets by (kernel, NMAX, UMAX); work items for every candidate tile size. The
+    // kernel (vector ALU or a matrix-core precision) is the one the instance's own handle runs,
+    // so only launch shapes of the same arithmetic compete.
+    std::map<std::array<int, 3>, std::vector<int>> dBuckets;
     for ( int i = 0; i < n; i++ )
-        dBuckets[ { nmaxOf( det[ i ].N ), roundUpPow2Users( det[ i ].U ) } ].push_back( i );
+        dBuckets[ { detectKernelOf( B->inst[ i ]->p.detect_precision, (uint32_t)det[ i ].N ), nmaxOf( det[ i ].N ),
+                    roundUpPow2Users( det[ i ].U ) } ]
+            .push_back( i );
     const size_t detOff = arena.add( det );
     struct DetBucket
     {
-        int nmax, umax;
+        int kernel, nmax, umax, nCand;
+        int threads[ 3 ];
         size_t itemOff[ 3 ];
         int nItems[ 3 ];
     };
@@ -514,13 +516,17 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
     for ( auto& kv : dBuckets )
     {
         DetBucket db {};
-        db.nmax = kv.first.first;
-        db.umax = kv.first.second;
-        for ( int c = 0; c < 3; c++ )
+        db.kernel = kv.first[ 0 ];
+        db.nmax = kv.first[ 1 ];
+        db.umax = kv.first[ 2 ];
+        db.nCand = db.kernel < 0 ? 3 : 1;
+        for ( int c = 0; c < db.nCand; c++ )
         {
+            db.threads[ c ] = db.kernel < 0 ? kTuneThreads[ c ] : apsm::kMfmaThreads;
+            const int tileSamples = db.kernel < 0 ? kTuneThreads[ c ] : apsm::kMfmaSamples;
             std::vector<apsm::WorkItem> items;
             for ( int i : kv.second )
-                for ( int64_t t = 0; t < ( det[ i ].M + kTuneThreads[ c ] - 1 ) / kTuneThreads[ c ]; t++ )
+                for ( int64_t t = 0; t < ( det[ i ].M + tileSamples - 1 ) / tileSamples; t++ )
                     items.push_back( apsm::WorkItem { i, (int)t } );
             db.itemOff[ c ] = arena.add( items );
             db.nItems[ c ] = (int)items.size();
@@ -549,9 +555,17 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
     HIP_OR_THROW( hipEventCreate( &t1 ) );
     for ( auto& db : dbs )
     {
+        const apsm::DetDesc* dd = (const apsm::DetDesc*)( base + detOff );
+        const int kernel = db.kernel, nmax = db.nmax, umax = db.umax;
+        auto launch = [ = ]( int threads, const apsm::WorkItem* it, int nItems, int grid ) {
+            if ( kernel < 0 )
+                launchBatchDetectAny( B->stream, nmax, umax, threads, dd, it, nItems, grid );
+            else
+                HIP_OR_THROW( apsm::launchBatchDetectMfma( kernel, nmax, umax, B->stream, dd, it, nItems, grid ) );
+        };
         float best = 1e30f;
-        int bc = 1, bb = 4;
-        for ( int c = 0; c < 3; c++ )
+        int bc = 0, bb = 4;
+        for ( int c = 0; c < db.nCand; c++ )
             for ( int bpc : kTuneBlocksPerCu )
             {
                 const int grid = std::max( 1, std::min( db.nItems[ c ], B->cus * bpc ) );
@@ -559,8 +573,7 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
                 for ( int rep = 0; rep < 2; rep++ )
                 {
                     HIP_OR_THROW( hipEventRecord( t0, B->stream ) );
-                    launchBatchDetectAny( B->stream, db.nmax, db.umax, kTuneThreads[ c ], (const apsm::DetDesc*)( base + detOff ),
-                                          (const apsm::WorkItem*)( base + db.itemOff[ c ] ), db.nItems[ c ], grid );
+                    launch( db.threads[ c ], (const apsm::WorkItem*)( base + db.itemOff[ c ] ), db.nItems[ c ], grid );
                     HIP_OR_THROW( hipEventRecord( t1, B->stream ) );
                     HIP_OR_THROW( hipEventSynchronize( t1 ) );
                     float t = 0.f;
@@ -574,13 +587,11 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
                     bb = bpc;
                 }
             }
-        const int threads = kTuneThreads[ bc ], nItems = db.nItems[ bc ];
+        const int threads = db.threads[ bc ], nItems = db.nItems[ bc ];
         const int grid = std::max( 1, std::min( nItems, B->cus * bb ) );
-        const apsm::DetDesc* dd = (const apsm::DetDesc*)( base + detOff );
         const apsm::WorkItem* it = (const apsm::WorkItem*)( base + db.itemOff[ bc ] );
-        const int nmax = db.nmax, umax = db.umax;
-        B->launches.push_back( [ = ]() { launchBatchDetectAny( B->stream, nmax, umax, threads, dd, it, nItems, grid ); } );
-        B->tuning.push_back( { nmax, umax, threads, bb, nItems } );
+        B->launches.push_back( [ = ]() { launch( threads, it, nItems, grid ); } );
+        B->tuning.push_back( { nmax, umax, threads, bb, nItems, kernel } );
     }
     (void)hipEventDestroy( t0 );
     (void)hipEventDestroy( t1 );

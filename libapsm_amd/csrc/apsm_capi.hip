@@ -9,6 +9,7 @@
 #include "apsm_train2.h"
 #include "apsm_train_nc.h"
 #include "apsm_batch.h"
+#include "apsm_detect_launch.h"
 
 #include <hip/hip_runtime.h>
 
@@ -128,6 +129,10 @@ struct StreamBuf
 };
 
 constexpr float kLog2e = 1.4426950408889634f;
+
+// APSM_PRECISION_FP32 runs the cross terms on the matrix cores from this many antennas on
+// (below, the vector-ALU kernel is at least as fast).
+constexpr uint32_t kMatrixFp32MinAntennas = 1;
 
 // Permutation of the reference shuffle (cpp/lib/noma/noma_detector.cu:59-74).
 std::vector<int32_t> shufflePermutation( int64_t T )
@@ -310,6 +315,7 @@ void validateParams( const apsm_params* p, uint32_t N, uint32_t U )
     default:
         throw ApsmError { APSM_ERR_ARG, "unknown detect version" };
     }
+    ARG_CHECK( p->detect_precision >= APSM_PRECISION_FP32 && p->detect_precision <= APSM_PRECISION_FP16, "unknown detect precision" );
 }
 
 // Blocked-path eligibility: single pass, no full-window start, window Gram fits the register
@@ -734,6 +740,23 @@ void launchDetectU( apsm_handle* h, const float* d_rx, int64_t M, float* d_est, 
         launchDetect<NMAX, 16>( h, d_rx, M, d_est, Z, Zn, C, ldC, theta, G, U );
 }
 
+// The detect kernel of a precision mode for N antennas: -1 for the vector-ALU kernel, else the
+// matrix-core input precision (apsm::PREC_*).
+int detectKernelOf( int32_t precision, uint32_t N )
+{
+    switch ( precision )
+    {
+    case APSM_PRECISION_FP32_VECTOR: return -1;
+    case APSM_PRECISION_FP32_MATRIX: return apsm::PREC_F32;
+    case APSM_PRECISION_BF16: return apsm::PREC_BF16;
+    case APSM_PRECISION_FP16: return apsm::PREC_F16;
+    default: return N >= kMatrixFp32MinAntennas ? apsm::PREC_F32 : -1;
+    }
+}
+
+int nmaxOf( int N ) { return N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64; }
+int roundUpPow2Users( int U ) { return U <= 1 ? 1 : U <= 2 ? 2 : U <= 4 ? 4 : U <= 8 ? 8 : 16; }
+
 void detectDev( apsm_handle* h, const float* d_rx, int64_t M, float* d_est )
 {
     ARG_CHECK( M > 0, "detect needs at least one sample" );
@@ -755,7 +778,15 @@ void detectDev( apsm_handle* h, const float* d_rx, int64_t M, float* d_est )
         const float* Cg = C ? C + (int64_t)u0 * ld : nullptr;
         const float* th = h->theta.ptr + (size_t)u0 * h->L;
         float* est = d_est + (size_t)u0 * M * 2;
-        if ( h->N <= 4 )
+        const int kernel = detectKernelOf( h->p.detect_precision, h->N );
+        if ( kernel >= 0 )
+        {
+            const float g2 = 0.5f / h->var * kLog2e;
+            HIP_OR_THROW( apsm::launchDetectMfma( kernel, nmaxOf( (int)h->N ), roundUpPow2Users( Ug ), h->stream, d_rx, M, Z, Zn, Cg, ld,
+                                                  th, G, Ug, h->p.linear_kernel_weight, h->p.gaussian_kernel_weight, g2, est,
+                                                  (int)h->N ) );
+        }
+        else if ( h->N <= 4 )
             launchDetectU<4>( h, d_rx, M, est, Z, Zn, Cg, ld, th, G, Ug );
         else if ( h->N <= 8 )
             launchDetectU<8>( h, d_rx, M, est, Z, Zn, Cg, ld, th, G, Ug );
@@ -802,6 +833,7 @@ void apsm_params_default( apsm_params* p )
     p->training_shuffle = 0;
     p->detect_version = APSM_VERSION_DETECT_BALANCED;
     p->linear_kernel_weight = 0.5f;
+    p->detect_precision = APSM_PRECISION_FP32;
 }
 
 const char* apsm_last_error( void ) { return g_lastError.c_str(); }
@@ -1573,8 +1605,8 @@ int apsm_batch_tuning( apsm_batch* B, int32_t* table, uint32_t cap_rows, uint32_
         {
             ARG_CHECK( cap_rows >= B->tuning.size(), "tuning table too small" );
             for ( size_t r = 0; r < B->tuning.size(); r++ )
-                for ( int c = 0; c < 5; c++ )
-                    table[ r * 5 + c ] = B->tuning[ r ][ c ];
+                for ( int c = 0; c < 6; c++ )
+                    table[ r * 6 + c ] = B->tuning[ r ][ c ];
         }
     } );
 }

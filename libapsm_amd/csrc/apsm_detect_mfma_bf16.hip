@@ -1,0 +1,11 @@
+// Matrix-core detect kernels with bf16 cross-term inputs, every antenna and user bound.
+#include "apsm_detect_mfma.h"
+
+namespace apsm {
+
+APSM_MFMA_ENTRY( mfmaEntryBF16 )
+{
+    return launchMfmaPrec<PREC_BF16>( nmax, umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
+}
+
+} // namespace apsm

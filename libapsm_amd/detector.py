@@ -27,6 +27,16 @@ APSM_VERSIONS = {
     "balanced": 0x10007,
 }
 
+# detectPrecision values (an extension of the reference configuration: arithmetic of the
+# detect cross terms; everything else stays FP32).
+APSM_PRECISIONS = {
+    "fp32": 0,
+    "fp32-vector": 1,
+    "fp32-matrix": 2,
+    "bf16": 3,
+    "fp16": 4,
+}
+
 _VERSION_STRINGS = {
     0x10001: "APSM Detect original",
     0x10002: "APSM Detect old fast code",
@@ -56,13 +66,15 @@ class apsm_parameters:  # noqa: N801 - reference class name
         self.normConstraint = 0.0
         self.eB = 0.001
         self.detectVersion = APSM_VERSIONS["balanced"]
+        self.detectPrecision = APSM_PRECISIONS["fp32"]
 
     def to_c(self):
         return ApsmParamsC(
             float(self.gaussianKernelWeight), float(self.gaussianKernelVariance), float(self.eB),
             float(self.normConstraint), int(self.windowSize), int(self.sampleStep), int(self.trainPasses),
             int(self.dictionarySize), int(bool(self.startWithFullWindow)), int(bool(self.llsInitialization)),
-            int(bool(self.trainingShuffle)), 0, int(self.detectVersion), float(self.linearKernelWeight))
+            int(bool(self.trainingShuffle)), 0, int(self.detectVersion), float(self.linearKernelWeight),
+            int(self.detectPrecision))
 
 
 def _rx_array(rx, num_antennas=None):
@@ -238,7 +250,8 @@ class ApsmNomaDetector(NomaDetector):
 class ApsmNomaDetectorFactory:
     VALID_KEYS = ("otype", "llsInitialization", "trainingShuffle", "gaussianKernelWeight",
                   "gaussianKernelVariance", "windowSize", "sampleStep", "trainPasses",
-                  "startWithFullWindow", "dictionarySize", "normConstraint", "eB", "detectVersion")
+                  "startWithFullWindow", "dictionarySize", "normConstraint", "eB", "detectVersion",
+                  "detectPrecision")
 
     @staticmethod
     def params_from_config(config):
@@ -262,6 +275,10 @@ class ApsmNomaDetectorFactory:
         if version not in APSM_VERSIONS:
             raise ValueError("Unknown detect version: " + str(version))
         par.detectVersion = APSM_VERSIONS[version]
+        precision = config.get("detectPrecision", "fp32")
+        if precision not in APSM_PRECISIONS:
+            raise ValueError("Unknown detect precision: " + str(precision))
+        par.detectPrecision = APSM_PRECISIONS[precision]
         return par
 
     @staticmethod

@@ -54,6 +54,7 @@ int main( int argc, char** argv )
                       << ", \"startWithFullWindow\": " << p.startWithFullWindow << ", \"llsInitialization\": " << p.llsInitialization
                       << ", \"trainingShuffle\": " << p.trainingShuffle << ", \"dictionarySize\": " << p.dictionarySize
                       << ", \"normConstraint\": " << p.normConstraint << ", \"eB\": " << p.eB << ", \"version\": " << v
+                      << ", \"detectPrecision\": " << p.detectPrecision
                       << ", \"versionString\": \"" << apsm_get_version_string( v ) << "\"}" << std::endl;
         }
         catch ( const std::invalid_argument& e )

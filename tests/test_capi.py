@@ -37,6 +37,8 @@ def test_params_default_matches_reference_factory():
     assert p.eB == pytest.approx(0.001)
     assert (p.window_size, p.sample_step, p.train_passes, p.dictionary_size) == (20, 2, 1, 685)
     assert p.detect_version == 0x10007
+    assert p.linear_kernel_weight == pytest.approx(0.5)
+    assert p.detect_precision == 0  # APSM_PRECISION_FP32
 
 
 @pytest.mark.parametrize("field,value,code", [
@@ -44,6 +46,8 @@ def test_params_default_matches_reference_factory():
     ("sample_step", 0, -1),
     ("window_size", 700, -1),
     ("detect_version", 12345, -1),
+    ("detect_precision", 5, -1),
+    ("detect_precision", -1, -1),
     ("norm_constraint", -1.0, -1),
     ("gaussian_kernel_weight", 1.5, -1),
 ])
@@ -90,6 +94,8 @@ def test_factory_validation_matches_reference():
         NomaDetectorFactory.build({"otype": "APSM", "windowsize": 3}, 8)
     with pytest.raises(ValueError, match="Unknown detect version"):
         NomaDetectorFactory.build({"otype": "APSM", "detectVersion": "fast"}, 8)
+    with pytest.raises(ValueError, match="Unknown detect precision"):
+        NomaDetectorFactory.build({"otype": "APSM", "detectPrecision": "fp8"}, 8)
     with pytest.raises(ValueError, match="Unknown algorithm"):
         NomaDetectorFactory.build({"otype": "KNN"}, 8)
     with pytest.raises(ValueError, match="Invalid key"):
@@ -97,3 +103,5 @@ def test_factory_validation_matches_reference():
     par = ApsmNomaDetectorFactory.params_from_config({"otype": "APSM", "gaussianKernelWeight": 0.25})
     assert par.linearKernelWeight == pytest.approx(0.75)
     assert par.dictionarySize == 685 and par.windowSize == 20 and par.sampleStep == 2
+    assert par.detectPrecision == 0
+    assert ApsmNomaDetectorFactory.params_from_config({"otype": "APSM", "detectPrecision": "fp16"}).detectPrecision == 4

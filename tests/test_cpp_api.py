@@ -37,7 +37,8 @@ def test_factory_defaults_and_overrides(api_check):
     for cfg in ({"otype": "APSM"},
                 {"otype": "APSM", "gaussianKernelWeight": 0.25, "windowSize": 80, "sampleStep": 4, "trainPasses": 2,
                  "startWithFullWindow": True, "llsInitialization": True, "trainingShuffle": True, "dictionarySize": 4096,
-                 "normConstraint": 1.5, "eB": 0.01, "gaussianKernelVariance": 0.25, "detectVersion": "shmem"}):
+                 "normConstraint": 1.5, "eB": 0.01, "gaussianKernelVariance": 0.25, "detectVersion": "shmem",
+                 "detectPrecision": "bf16"}):
         got = api_check("factory", json.dumps(cfg))
         ref = ApsmNomaDetectorFactory.params_from_config(cfg)
         for k in ("linearKernelWeight", "gaussianKernelWeight", "gaussianKernelVariance", "normConstraint", "eB"):
@@ -47,12 +48,14 @@ def test_factory_defaults_and_overrides(api_check):
         for k in ("startWithFullWindow", "llsInitialization", "trainingShuffle"):
             assert bool(got[k]) == bool(getattr(ref, k)), k
         assert got["version"] == ref.detectVersion
+        assert got["detectPrecision"] == ref.detectPrecision
     assert api_check("factory", '{"otype": "APSM"}')["versionString"] == "APSM Detect Balance memory and computation"
 
 
 def test_factory_errors(api_check):
     assert api_check("factory", '{"otype": "APSM", "window": 3}') == {"invalid_argument": "Invalid key: window"}
     assert api_check("factory", '{"otype": "APSM", "detectVersion": "fast"}') == {"invalid_argument": "Unknown detect version: fast"}
+    assert api_check("factory", '{"otype": "APSM", "detectPrecision": "fp8"}') == {"invalid_argument": "Unknown detect precision: fp8"}
 
 
 @pytest.mark.parametrize("bps", [1, 2, 4, 6])

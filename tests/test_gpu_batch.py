@@ -10,7 +10,7 @@ from helpers import to_ri
 pytestmark = pytest.mark.gpu
 
 
-def make_batch(count, seed):
+def make_batch(count, seed, modes=("fp32",)):
     from libapsm_amd import datagen
     rng = np.random.default_rng(seed)
     inst = []
@@ -24,7 +24,7 @@ def make_batch(count, seed):
         mod = str(rng.choice(["QPSK", "QAM16"]))
         d = datagen.generate(N, max(U, 3), T, M, modulation=mod, snr_db=15.0, seed=1000 + i)
         cfg = dict(windowSize=w, dictionarySize=dc, eB=0.01, gaussianKernelVariance=datagen.kernel_variance_for_sigma(0.5, N),
-                   sampleStep=int(rng.choice([2, 2, 4])))
+                   sampleStep=int(rng.choice([2, 2, 4])), detectPrecision=str(rng.choice(list(modes))))
         inst.append(dict(cfg=cfg, N=N, U=U, d=d, bps=d["bps"]))
     return inst
 
@@ -32,7 +32,7 @@ def make_batch(count, seed):
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_batch_equals_single_handles(require_gpu, use_graph):
     from libapsm_amd import ApsmBatch, ApsmNomaDetectorFactory
-    inst = make_batch(14, seed=5)
+    inst = make_batch(14, seed=5, modes=("fp32", "fp32-vector", "fp32-matrix", "bf16", "fp16"))
     b = ApsmBatch([x["cfg"] for x in inst], [x["N"] for x in inst], [x["U"] for x in inst])
     b.plan([x["d"]["rx_train"] for x in inst], [x["d"]["tx_train"][:x["U"]] for x in inst], [x["d"]["rx_data"] for x in inst],
            use_graph=use_graph)
@@ -46,6 +46,7 @@ def test_batch_equals_single_handles(require_gpu, use_graph):
     assert b.last_ms() > 0
     tun = b.tuning()
     assert tun and all(t["threads"] in (128, 256, 512) and t["blocks_per_cu"] in (2, 4, 8) for t in tun)
+    assert {t["kernel"] for t in tun} >= {"vector", "matrix-bf16"}
     for i, x in enumerate(inst):
         det = ApsmNomaDetectorFactory.build(dict(x["cfg"], otype="APSM"), x["N"], x["U"])
         det.train(x["d"]["rx_train"], x["d"]["tx_train"][:x["U"]])
@@ -56,7 +57,7 @@ def test_batch_equals_single_handles(require_gpu, use_graph):
 
 def test_batch_matches_oracle(require_gpu, oracle):
     from libapsm_amd import ApsmBatch
-    inst = make_batch(6, seed=8)
+    inst = make_batch(6, seed=8, modes=("fp32", "fp32-vector"))
     b = ApsmBatch([x["cfg"] for x in inst], [x["N"] for x in inst], [x["U"] for x in inst])
     b.plan([x["d"]["rx_train"] for x in inst], [x["d"]["tx_train"][:x["U"]] for x in inst], [x["d"]["rx_data"] for x in inst])
     b.synchronize()

@@ -1,0 +1,82 @@
+"""Detect arithmetic modes (detectPrecision) against the FP64 oracle.
+
+* fp32-vector and fp32-matrix (cross terms on the vector ALUs or on the matrix cores, FP32
+  inputs and accumulation): the FP32 parity bar of test_gpu_parity (soft outputs within 2e-3,
+  hard decisions equal away from decision boundaries), for every antenna bucket and for user
+  groups that fill and overflow a 16-user launch.
+* bf16 and fp16 (cross-term inputs rounded to 16 bits, FP32 accumulation, FP32 norms,
+  exponentials and contraction): the soft-output RMS error relative to the RMS soft output
+  and the symbol-error count against the oracle's, with the bounds written below.
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+from helpers import small_problem, to_ri
+from test_gpu_parity import compare, orc_run
+
+pytestmark = pytest.mark.gpu
+
+FP32_MODES = ["fp32-vector", "fp32-matrix"]
+# relative soft-output RMS error bound and relative SER deviation bound (plus 3 symbols of slack)
+HALF_BOUNDS = {"bf16": (0.05, 0.05), "fp16": (0.01, 0.01)}
+
+
+def run_mode(cfg, N, users, d, mode):
+    from libapsm_amd import ApsmNomaDetectorFactory
+    det = ApsmNomaDetectorFactory.build(dict(cfg, otype="APSM", detectPrecision=mode), N, users)
+    det.train(d["rx_train"], d["tx_train"][:users] if users > 1 else d["tx_train"][0])
+    est = det.detect(d["rx_data"])
+    return est[None, :] if users == 1 else est
+
+
+@pytest.mark.parametrize("mode", FP32_MODES)
+@pytest.mark.parametrize("N", [3, 4, 8, 13, 16, 32, 64])
+def test_fp32_modes_match_oracle(require_gpu, oracle, mode, N):
+    print(f"mode {mode} N={N}", flush=True)
+    cfg = dict(windowSize=20, dictionarySize=300, eB=0.01, gaussianKernelVariance=0.25 / (N * N))
+    d = small_problem(N=N, U=3, T=300, M=777, seed=100 + N)
+    est = run_mode(cfg, N, 1, d, mode)
+    _, est_o = orc_run(oracle, cfg, N, d["rx_train"], d["tx_train"][0], d["rx_data"])
+    compare(est[0], est_o, 2)
+
+
+@pytest.mark.parametrize("mode", FP32_MODES + ["bf16", "fp16"])
+@pytest.mark.parametrize("users", [5, 19])
+def test_user_groups_equal_single_user_handles(require_gpu, mode, users):
+    """Users are launched in groups of up to 16 with the kernel instantiated for the group's
+    padded user count; every user still gets exactly what a one-user handle computes."""
+    print(f"mode {mode} users={users}", flush=True)
+    cfg = dict(windowSize=16, dictionarySize=200, eB=0.01, gaussianKernelVariance=0.25 / 64)
+    d = small_problem(N=8, U=users, T=250, M=300, seed=users)
+    est = run_mode(cfg, 8, users, d, mode)
+    for u in (0, users // 2, users - 1):
+        one = dict(d, tx_train=d["tx_train"][u:u + 1])
+        np.testing.assert_array_equal(est[u], run_mode(cfg, 8, 1, one, mode)[0], err_msg=f"user {u}")
+
+
+@pytest.mark.parametrize("mode", ["bf16", "fp16"])
+@pytest.mark.parametrize("N,mod", [(8, "QAM16"), (16, "QAM16"), (64, "QPSK")])
+def test_half_precision_error_vs_oracle(require_gpu, oracle, mode, N, mod):
+    from libapsm_amd import datagen
+    print(f"mode {mode} N={N} {mod}", flush=True)
+    bps = datagen.BITS_PER_SYMBOL[mod]
+    cfg = dict(windowSize=40, dictionarySize=800, eB=0.01, gaussianKernelVariance=datagen.kernel_variance_for_sigma(0.5, N))
+    d = small_problem(N=N, U=4, T=1500, M=12000, seed=zlib.crc32(f"{mode}{N}".encode()) % 1000, mod=mod, snr_db=15.0)
+    est = run_mode(cfg, N, 1, d, mode)[0]
+    ref32 = run_mode(cfg, N, 1, d, "fp32")[0]
+    _, est_o = orc_run(oracle, cfg, N, d["rx_train"], d["tx_train"][0], d["rx_data"])
+    g = to_ri(est).astype(np.float64)
+    rms_o = np.sqrt(np.mean(est_o ** 2))
+    rel_rms = np.sqrt(np.mean((g - est_o) ** 2)) / rms_o
+    rel_rms32 = np.sqrt(np.mean((to_ri(ref32).astype(np.float64) - est_o) ** 2)) / rms_o
+    tx = to_ri(d["tx_data"][0])
+    e_p = oracle.symbol_errors(g, tx, bps)
+    e_o = oracle.symbol_errors(est_o, tx, bps)
+    e_32 = oracle.symbol_errors(to_ri(ref32), tx, bps)
+    print(f"{mode} N={N} {mod}: rel soft RMS err {rel_rms:.3e} (fp32 {rel_rms32:.3e}), max {np.abs(g - est_o).max():.3e}; "
+          f"symbol errors {e_p} (fp32 {e_32}, oracle {e_o}) of {tx.shape[0]}")
+    rms_bound, ser_bound = HALF_BOUNDS[mode]
+    assert rel_rms <= rms_bound
+    assert abs(e_p - e_o) <= ser_bound * e_o + 3
