@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--no-lls", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-instances measurement")
     ap.add_argument("--batched-instances", type=int, default=512)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision large-instance measurement")
+    ap.add_argument("--mixed-samples", type=int, default=None, help="override the mixed workload's detect samples")
     return ap.parse_args()
 
 
@@ -191,6 +193,104 @@ def bench_batched(count, steps):
     }
 
 
+def fp64_detect(x, dict_c, coeffs, theta, var, wL, wG):
+    """FP64 evaluation of a trained detector's output for samples x [N][S] (complex):
+    wL theta.x + wG sum_j (c_a K + ...) with the complex-pair kernel terms
+    E0 = exp(-(|x|^2 + |z|^2 - 2p) / 2var), E1 = ...(- 2s), E2 = ...(+ 2s), p = Re(x.conj z),
+    s = -Im(x.conj z). Returns [U][S][2]."""
+    x = x.astype(np.complex128)
+    z = dict_c.astype(np.complex128)
+    N = x.shape[0]
+    th = theta.astype(np.float64)
+    c = coeffs.astype(np.float64)
+    xn = np.sum(np.abs(x) ** 2, axis=0)
+    zn = np.sum(np.abs(z) ** 2, axis=0)
+    xz = x.T @ np.conj(z)  # [S][G]
+    p, sp = xz.real, -xz.imag
+    nn = xn[:, None] + zn[None, :]
+    E0, E1, E2 = (np.exp(-np.maximum(nn - 2 * p, 0) / (2 * var)), np.exp(-np.maximum(nn - 2 * sp, 0) / (2 * var)),
+                  np.exp(-np.maximum(nn + 2 * sp, 0) / (2 * var)))
+    out = np.zeros((th.shape[0], x.shape[1], 2))
+    for u in range(th.shape[0]):
+        ca, cb = c[u, 0::2], c[u, 1::2]
+        la = th[u, :N] @ x.real + th[u, N:] @ x.imag
+        lb = th[u, :N] @ x.imag - th[u, N:] @ x.real
+        out[u, :, 0] = wL * la + wG * (E0 @ ca + E1 @ cb)
+        out[u, :, 1] = wL * lb + wG * (E2 @ ca + E0 @ cb)
+    return out
+
+
+def bench_mixed(steps, samples=None):
+    """Mixed-precision large instance: 64 antennas, 12 QPSK users at 12 dB, dictionary of
+    16384 Gaussian centres (16384 pilots), 8M detect samples, the reference's default kernel
+    variance. One train, then the detect of all users in FP32 (matrix cores), BF16 and FP16
+    cross terms. Per dtype: detect time and symbols/s, symbol errors over all samples, and the
+    soft-output RMS error against an FP64 evaluation of the same trained detector on the first
+    4096 samples."""
+    from libapsm_amd import ApsmNomaDetectorFactory, datagen
+    from libapsm_amd._lib import DeviceArray
+
+    w = dict(datagen.WORKLOADS["mixed-64ant-qpsk"])
+    N, U, T = w["N"], len(w["users"]), w["T"]
+    M = samples or w["M"]
+    t0 = time.perf_counter()
+    data = datagen.generate(N, w["U_tx"], T, M, modulation=w["modulation"], snr_db=w["snr_db"], seed=w["seed"])
+    t_gen = time.perf_counter() - t0
+    up = lambda z: DeviceArray(np.ascontiguousarray(z, dtype=np.complex64).view(np.float32))  # noqa: E731
+    d_rxt, d_txt, d_rxd = up(data["rx_train"]), up(data["tx_train"][w["users"]]), up(data["rx_data"])
+    d_est = DeviceArray.empty((U, M, 2), np.float32)
+    cfg = {"otype": "APSM", "windowSize": w["window"], "eB": w["eB"], "dictionarySize": w["dict_size"],
+           "gaussianKernelVariance": 0.05}
+    det = ApsmNomaDetectorFactory.build(cfg, N, U)
+    det.train_device(d_rxt.ptr, d_txt.ptr, T)
+    det.synchronize()
+    train_ms = det.kernel_times()[0]
+    st = det.getTrainingState()
+    dict_c = det.getDictionary()
+    G = dict_c.shape[1]
+    sub = min(4096, M)
+    ref = fp64_detect(data["rx_data"][:, :sub], dict_c, st["coeffs"], st["theta"], st["variance"],
+                      det.par.linearKernelWeight, det.par.gaussianKernelWeight)
+    rms_ref = float(np.sqrt(np.mean(ref ** 2)))
+    tx = data["tx_data"][w["users"]]
+    ref_c = ref[..., 0] + 1j * ref[..., 1]
+    e_ref = datagen.symbol_errors(ref_c, tx[:, :sub], data["bps"])
+    out = {"workload": "mixed-64ant-qpsk", "antennas": N, "users": U, "dictionary": G, "detect_samples": M,
+           "pilots": T, "gaussianKernelVariance": 0.05, "train_ms": train_ms, "datagen_s": t_gen,
+           "fp64_subsample": sub, "fp64_symbol_errors_subsample": int(e_ref), "dtypes": {}}
+    for mode in ("fp32", "bf16", "fp16"):
+        det.setDetectPrecision(mode)
+        det.detect_device(d_rxd.ptr, M, d_est.ptr)
+        det.synchronize()
+        ms = []
+        for _ in range(steps):
+            det.detect_device(d_rxd.ptr, M, d_est.ptr)
+            det.synchronize()
+            ms.append(det.kernel_times()[1])
+        est = d_est.numpy()
+        est_c = est[..., 0] + 1j * est[..., 1]
+        errs = datagen.symbol_errors(est_c, tx, data["bps"])
+        e_sub = datagen.symbol_errors(est_c[:, :sub], tx[:, :sub], data["bps"])
+        det_ms = float(np.mean(ms))
+        out["dtypes"][mode] = {
+            "detect_ms": det_ms,
+            "symbols_per_s": M * U / (det_ms / 1000.0),
+            "gaussian_pairs_per_s": M * G / (det_ms / 1000.0),
+            "ser": errs / est_c.size,
+            "symbol_errors": int(errs),
+            "soft_rms_err_vs_fp64": float(np.sqrt(np.mean((est[:, :sub].astype(np.float64) - ref) ** 2))),
+            "soft_rel_rms_err_vs_fp64": float(np.sqrt(np.mean((est[:, :sub].astype(np.float64) - ref) ** 2)) / rms_ref),
+            "soft_max_err_vs_fp64": float(np.abs(est[:, :sub].astype(np.float64) - ref).max()),
+            "symbol_errors_subsample": int(e_sub),
+        }
+    f32 = out["dtypes"]["fp32"]
+    for mode in ("bf16", "fp16"):
+        m = out["dtypes"][mode]
+        m["ser_rel_diff_vs_fp32"] = abs(m["symbol_errors"] - f32["symbol_errors"]) / max(1, f32["symbol_errors"])
+        m["speedup_vs_fp32"] = f32["detect_ms"] / m["detect_ms"]
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -294,9 +394,12 @@ def main():
                       "apsm_ser_same_user": datagen.symbol_errors(est_c[0], tx_data[0], data["bps"]) / M}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, data, args.cpu_seconds)
-    if not args.no_batched:
+    if not args.no_batched or not args.no_mixed:
         del d_rxt, d_txt, d_rxd, d_est, det
+    if not args.no_batched:
         out["batched"] = bench_batched(args.batched_instances, max(1, args.steps))
+    if not args.no_mixed:
+        out["mixed_precision"] = bench_mixed(max(1, args.steps), args.mixed_samples)
     print(json.dumps(out))
 
 

@@ -133,6 +133,13 @@ public:
     void resetState() override { libapsm::check( apsm_reset( handle ), "ApsmNomaDetector::resetState" ); }
 
     // (w_L |theta|^2) / (w_G c^T K c) of user 0 (apsm_noma_detector.cu:220-319)
+    // Detect arithmetic for the following detects (APSM_PRECISION_*); keeps the trained state.
+    void setDetectPrecision( int32_t precision )
+    {
+        libapsm::check( apsm_set_detect_precision( handle, precision ), "ApsmNomaDetector::setDetectPrecision" );
+        par.detectPrecision = precision;
+    }
+
     RealSample computeNormRatio( uint32_t user = 0 )
     {
         float r = 0.f;

@@ -137,6 +137,10 @@ int apsm_get_state(apsm_handle* h, uint64_t* used_rows, uint32_t* dict_entries, 
  * first. dict may be NULL to query *dict_entries only. */
 int apsm_get_dictionary(apsm_handle* h, float* dict, uint32_t cap_entries, uint32_t* dict_entries);
 
+/* Switches the detect arithmetic of an existing handle (APSM_PRECISION_*); the trained state
+ * is kept. No counterpart in the reference (its detect arithmetic is fixed). */
+int apsm_set_detect_precision(apsm_handle* h, int32_t precision);
+
 /* Replaces ApsmNomaDetector::computeNormRatio (cpp/lib/apsm/apsm_noma_detector.cu:220-319):
  * (w_L |theta|^2) / (w_G c^T K c) over the live dictionary, for one user of the handle. */
 int apsm_norm_ratio(apsm_handle* h, uint32_t user, float* ratio);

@@ -68,6 +68,7 @@ SIGNATURES = {
     "lls_synchronize": (ctypes.c_int, [_vp]),
     "apsm_get_dictionary": (ctypes.c_int, [_vp, _vp, _u32, _P(ctypes.c_uint32)]),
     "apsm_norm_ratio": (ctypes.c_int, [_vp, _u32, _f32p]),
+    "apsm_set_detect_precision": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "apsm_train_history": (ctypes.c_int, [_vp, _vp, _vp, _u32, _vp, _vp, _u32, _u32, ctypes.c_float, _vp, _u32,
                                           _P(ctypes.c_uint32), _f32p]),
     "lls_create_on_stream": (ctypes.c_int, [_u32, _vp, _P(_vp)]),

@@ -1108,6 +1108,15 @@ int apsm_get_dictionary( apsm_handle* h, float* dict, uint32_t cap_entries, uint
     } );
 }
 
+int apsm_set_detect_precision( apsm_handle* h, int32_t precision )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h != nullptr, "null handle" );
+        ARG_CHECK( precision >= APSM_PRECISION_FP32 && precision <= APSM_PRECISION_FP16, "unknown detect precision" );
+        h->p.detect_precision = precision;
+    } );
+}
+
 int apsm_norm_ratio( apsm_handle* h, uint32_t user, float* ratio )
 {
     return guarded( [ & ] {

@@ -212,6 +212,13 @@ class ApsmNomaDetector(NomaDetector):
         self.last_detect_ms = ms.value
         return est[0] if self.numUsers == 1 else est
 
+    def setDetectPrecision(self, precision):
+        """Detect arithmetic for the following detects ("fp32", "fp32-vector", "fp32-matrix",
+        "bf16", "fp16"); the trained state is kept."""
+        if precision not in APSM_PRECISIONS:
+            raise ValueError("Unknown detect precision: " + str(precision))
+        check(lib().apsm_set_detect_precision(self._h, APSM_PRECISIONS[precision]))
+
     def train_device(self, d_rx, d_tx, T):
         """Device pointers (ints) in the host layouts; enqueued on the handle's stream."""
         check(lib().apsm_train_dev(self._h, ctypes.c_void_p(d_rx), ctypes.c_void_p(d_tx), int(T)))

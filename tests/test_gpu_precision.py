@@ -57,13 +57,18 @@ def test_user_groups_equal_single_user_handles(require_gpu, mode, users):
 
 
 @pytest.mark.parametrize("mode", ["bf16", "fp16"])
-@pytest.mark.parametrize("N,mod", [(8, "QAM16"), (16, "QAM16"), (64, "QPSK")])
-def test_half_precision_error_vs_oracle(require_gpu, oracle, mode, N, mod):
+@pytest.mark.parametrize("N,mod,var", [(8, "QAM16", 0.05), (8, "QAM16", "sigma0.5"), (16, "QAM16", 0.05), (64, "QPSK", 0.05),
+                                       (64, "QPSK", "sigma0.5")])
+def test_half_precision_error_vs_oracle(require_gpu, oracle, mode, N, mod, var):
+    """Six superimposed users at 12 dB so that symbol errors occur; the reference's default
+    kernel variance (a wide kernel: every dictionary entry contributes) and sigma = 0.5 after
+    power normalisation (a narrow kernel)."""
     from libapsm_amd import datagen
-    print(f"mode {mode} N={N} {mod}", flush=True)
+    print(f"mode {mode} N={N} {mod} var={var}", flush=True)
     bps = datagen.BITS_PER_SYMBOL[mod]
-    cfg = dict(windowSize=40, dictionarySize=800, eB=0.01, gaussianKernelVariance=datagen.kernel_variance_for_sigma(0.5, N))
-    d = small_problem(N=N, U=4, T=1500, M=12000, seed=zlib.crc32(f"{mode}{N}".encode()) % 1000, mod=mod, snr_db=15.0)
+    v = datagen.kernel_variance_for_sigma(0.5, N) if var == "sigma0.5" else var
+    cfg = dict(windowSize=40, dictionarySize=800, eB=0.01, gaussianKernelVariance=v)
+    d = small_problem(N=N, U=6, T=1500, M=12000, seed=zlib.crc32(f"{mode}{N}{var}".encode()) % 1000, mod=mod, snr_db=12.0)
     est = run_mode(cfg, N, 1, d, mode)[0]
     ref32 = run_mode(cfg, N, 1, d, "fp32")[0]
     _, est_o = orc_run(oracle, cfg, N, d["rx_train"], d["tx_train"][0], d["rx_data"])
@@ -75,8 +80,23 @@ def test_half_precision_error_vs_oracle(require_gpu, oracle, mode, N, mod):
     e_p = oracle.symbol_errors(g, tx, bps)
     e_o = oracle.symbol_errors(est_o, tx, bps)
     e_32 = oracle.symbol_errors(to_ri(ref32), tx, bps)
-    print(f"{mode} N={N} {mod}: rel soft RMS err {rel_rms:.3e} (fp32 {rel_rms32:.3e}), max {np.abs(g - est_o).max():.3e}; "
-          f"symbol errors {e_p} (fp32 {e_32}, oracle {e_o}) of {tx.shape[0]}")
+    print(f"{mode} N={N} {mod} var={var}: rel soft RMS err {rel_rms:.3e} (fp32 {rel_rms32:.3e}), "
+          f"max {np.abs(g - est_o).max():.3e}; symbol errors {e_p} (fp32 {e_32}, oracle {e_o}) of {tx.shape[0]}")
     rms_bound, ser_bound = HALF_BOUNDS[mode]
     assert rel_rms <= rms_bound
     assert abs(e_p - e_o) <= ser_bound * e_o + 3
+
+
+def test_switching_precision_keeps_the_trained_state(require_gpu):
+    """setDetectPrecision on a trained handle gives what a handle created with that precision
+    computes after the same train (the train does not depend on the detect arithmetic)."""
+    from libapsm_amd import ApsmNomaDetectorFactory
+    cfg = dict(windowSize=16, dictionarySize=300, eB=0.01, gaussianKernelVariance=0.05)
+    d = small_problem(N=16, U=4, T=400, M=2000, seed=77)
+    det = ApsmNomaDetectorFactory.build(dict(cfg, otype="APSM"), 16)
+    det.train(d["rx_train"], d["tx_train"][0])
+    for mode in ["bf16", "fp32-vector", "fp16", "fp32-matrix"]:
+        det.setDetectPrecision(mode)
+        np.testing.assert_array_equal(det.detect(d["rx_data"]), run_mode(cfg, 16, 1, d, mode)[0], err_msg=mode)
+    with pytest.raises(ValueError):
+        det.setDetectPrecision("int8")
