@@ -32,6 +32,7 @@ extern "C" {
 #define APSM_ERR_DEVICE -2 /* HIP runtime failure (reference: CUDA_CHECK -> exit) */
 #define APSM_ERR_NUMERIC -3 /* least-squares system not positive definite */
 #define APSM_ERR_UNSUPPORTED -4 /* configuration the device path does not implement */
+#define APSM_ERR_BUSY -5 /* admission refused: the HBM budget cannot hold it */
 
 /* detectVersion values (cpp/lib/apsm/apsm_versions.cuh:75-89). All four select the same
  * dictionary-streaming kernel here; the names are accepted for drop-in compatibility. */
@@ -225,6 +226,69 @@ apsm_handle* apsm_batch_instance(apsm_batch* b, uint32_t i);
 /* Autotuner choices: rows of (NMAX, UMAX, threads per workgroup, workgroups per CU, work items,
  * kernel: -1 vector ALU, else the matrix-core input precision 0 FP32, 1 BF16, 2 FP16). */
 int apsm_batch_tuning(apsm_batch* b, int32_t* table, uint32_t cap_rows, uint32_t* rows);
+
+/* Serving engine (no counterpart in the reference, whose tool detects one data set per run).
+ * A server keeps links -- detectors whose state stays on the device between frames -- and
+ * serves frames of them: detect the frame's data samples with the link's current state, then
+ * train the link incrementally on the frame's pilots (the next frame of the link sees the
+ * update). Frames of different links are served by continuous batching: every round detects
+ * the frames admitted since the previous round in batched launches and advances every link
+ * train in progress by one block, so frames that arrive while others are on the GPU join
+ * the next round. Each result equals what apsm_detect followed by apsm_train on a handle of
+ * its own computes (same kernels, same order). Links are admitted against hbm_budget (their
+ * buffers are reserved for max_pilots pilots per frame; APSM_ERR_BUSY when it is spent) and
+ * frames against the frame arena (a frame waits for room, in arrival order). Link
+ * requirements are those of apsm_batch. Frame buffers are host memory (pinned for overlap);
+ * est is written when the frame's result is reported. */
+typedef struct apsm_server apsm_server;
+typedef struct apsm_server_config
+{
+    uint64_t hbm_budget; /* device bytes for link states and the frame arena together */
+    uint64_t frame_arena; /* device bytes of the frame arena (part of hbm_budget) */
+} apsm_server_config;
+typedef struct apsm_frame
+{
+    double arrival_s; /* apsm_server_replay: arrival time relative to the start of the replay */
+    uint32_t link;
+    uint32_t n_data; /* samples to detect */
+    uint32_t n_pilots; /* pilots trained on after the detect (0: no train) */
+    uint32_t reserved;
+    const float* rx_data; /* [N][n_data][2] */
+    const float* rx_pilots; /* [N][n_pilots][2] */
+    const float* tx_pilots; /* [U][n_pilots][2] */
+    float* est; /* [U][n_data][2] */
+} apsm_frame;
+typedef struct apsm_frame_result
+{
+    int32_t status;
+    uint32_t reserved;
+    uint64_t ticket;
+    double arrival_s, admit_s; /* seconds on the server clock (replay: from its start) */
+    double output_s; /* estimates in est (the frame's latency is output_s - arrival_s) */
+    double trained_s; /* the link's train on the frame's pilots finished */
+    int64_t detect_round, last_round;
+} apsm_frame_result;
+int apsm_server_create(const apsm_server_config* cfg, apsm_server** out);
+void apsm_server_destroy(apsm_server* s);
+int apsm_server_add_link(apsm_server* s, const apsm_params* p, uint32_t num_antennas, uint32_t num_users, uint32_t max_pilots,
+                         uint32_t* link);
+/* The link's detector handle (owned by the server) for state readback. */
+apsm_handle* apsm_server_link(apsm_server* s, uint32_t link);
+/* Queues a frame (arrival = now); *ticket identifies its result. */
+int apsm_server_submit(apsm_server* s, const apsm_frame* frame, uint64_t* ticket);
+/* Makes progress (retires finished rounds, launches the next) and reports finished frames,
+ * up to cap; wait != 0 keeps going until every submitted frame has finished. */
+int apsm_server_poll(apsm_server* s, int wait, apsm_frame_result* out, uint32_t cap, uint32_t* n);
+/* Serves n frames arriving at frames[i].arrival_s (non-decreasing) after the call starts, on
+ * the host's clock, and returns when all have finished; results[i] belongs to frames[i]. */
+int apsm_server_replay(apsm_server* s, const apsm_frame* frames, uint32_t n, apsm_frame_result* results);
+/* Rounds launched so far, the largest number of frames one round carried, device bytes of
+ * the links. */
+int apsm_server_stats(apsm_server* s, uint64_t* rounds, uint64_t* max_frames_per_round, uint64_t* link_bytes);
+
+/* Pinned host memory (for frame buffers whose copies overlap the device work). */
+int apsm_host_malloc(void** ptr, uint64_t bytes);
+int apsm_host_free(void* ptr);
 
 /* CudaStream (cpp/lib/cuda/cuda_stream.cuh:67-96): a HIP stream of default priority. */
 int apsm_stream_create(void** out);

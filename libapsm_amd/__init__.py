@@ -11,8 +11,9 @@ Array conventions (cpp/lib/noma/types.cuh:69-100): received signals are complex
 [antenna][sample]; pilots are complex [sample] (or [user][sample] for a multi-user
 handle); estimates come back as complex64 [sample] (or [user][sample]).
 """
-from ._lib import ApsmError, DeviceArray, lib, library_path  # noqa: F401
+from ._lib import ApsmError, DeviceArray, PinnedArray, lib, library_path  # noqa: F401
 from .batch import ApsmBatch  # noqa: F401
+from .serve import ApsmServer, Frame  # noqa: F401
 from .detector import (  # noqa: F401
     APSM_VERSIONS,
     ApsmNomaDetector,

@@ -38,6 +38,38 @@ class ApsmParamsC(ctypes.Structure):
     ]
 
 
+class ServerConfigC(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("frame_arena", ctypes.c_uint64)]
+
+
+class FrameC(ctypes.Structure):
+    _fields_ = [
+        ("arrival_s", ctypes.c_double),
+        ("link", ctypes.c_uint32),
+        ("n_data", ctypes.c_uint32),
+        ("n_pilots", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("rx_data", ctypes.c_void_p),
+        ("rx_pilots", ctypes.c_void_p),
+        ("tx_pilots", ctypes.c_void_p),
+        ("est", ctypes.c_void_p),
+    ]
+
+
+class FrameResultC(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32),
+        ("ticket", ctypes.c_uint64),
+        ("arrival_s", ctypes.c_double),
+        ("admit_s", ctypes.c_double),
+        ("output_s", ctypes.c_double),
+        ("trained_s", ctypes.c_double),
+        ("detect_round", ctypes.c_int64),
+        ("last_round", ctypes.c_int64),
+    ]
+
+
 # Every symbol include/apsm_capi.h declares, with (restype, argtypes).
 _P = ctypes.POINTER
 _vp = ctypes.c_void_p
@@ -69,6 +101,16 @@ SIGNATURES = {
     "apsm_get_dictionary": (ctypes.c_int, [_vp, _vp, _u32, _P(ctypes.c_uint32)]),
     "apsm_norm_ratio": (ctypes.c_int, [_vp, _u32, _f32p]),
     "apsm_set_detect_precision": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "apsm_server_create": (ctypes.c_int, [_P(ServerConfigC), _P(_vp)]),
+    "apsm_server_destroy": (None, [_vp]),
+    "apsm_server_add_link": (ctypes.c_int, [_vp, _P(ApsmParamsC), _u32, _u32, _u32, _P(ctypes.c_uint32)]),
+    "apsm_server_link": (_vp, [_vp, _u32]),
+    "apsm_server_submit": (ctypes.c_int, [_vp, _P(FrameC), _P(ctypes.c_uint64)]),
+    "apsm_server_poll": (ctypes.c_int, [_vp, ctypes.c_int, _P(FrameResultC), _u32, _P(ctypes.c_uint32)]),
+    "apsm_server_replay": (ctypes.c_int, [_vp, _P(FrameC), _u32, _P(FrameResultC)]),
+    "apsm_server_stats": (ctypes.c_int, [_vp, _P(ctypes.c_uint64), _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
+    "apsm_host_malloc": (ctypes.c_int, [_P(_vp), ctypes.c_uint64]),
+    "apsm_host_free": (ctypes.c_int, [_vp]),
     "apsm_train_history": (ctypes.c_int, [_vp, _vp, _vp, _u32, _vp, _vp, _u32, _u32, ctypes.c_float, _vp, _u32,
                                           _P(ctypes.c_uint32), _f32p]),
     "lls_create_on_stream": (ctypes.c_int, [_u32, _vp, _P(_vp)]),
@@ -120,6 +162,31 @@ def lib():
 def check(rc):
     if rc != APSM_OK:
         raise ApsmError(rc, lib().apsm_last_error().decode())
+
+
+class PinnedArray:
+    """A numpy array in pinned host memory of the library's HIP runtime (so that the device
+    copies of frames overlap the device work). .array is the numpy view."""
+
+    def __init__(self, shape, dtype):
+        import numpy as np
+        self.shape, self.dtype = tuple(shape), np.dtype(dtype)
+        self.nbytes = max(1, int(np.prod(self.shape)) * self.dtype.itemsize)
+        self._p = ctypes.c_void_p()
+        check(lib().apsm_host_malloc(ctypes.byref(self._p), self.nbytes))
+        buf = (ctypes.c_char * self.nbytes).from_address(self._p.value)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=int(np.prod(self.shape))).reshape(self.shape)
+
+    @property
+    def ptr(self):
+        return self._p.value or 0
+
+    def __del__(self):
+        p = getattr(self, "_p", None)
+        if p is not None and p.value and _lib is not None:
+            self.array = None
+            lib().apsm_host_free(p)
+            self._p = ctypes.c_void_p()
 
 
 class DeviceArray:

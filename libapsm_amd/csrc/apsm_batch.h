@@ -115,6 +115,27 @@ __global__ void __launch_bounds__( THREADS ) batch_detect( const DetDesc* __rest
     }
 }
 
+// Row-wise copies between device buffers: rows x n floats from src (row stride ldSrc) to dst
+// (row stride ldDst); the buffers do not overlap.
+struct CopyDesc
+{
+    const float* src;
+    float* dst;
+    int64_t n, ldSrc, ldDst;
+    int rows;
+};
+
+__global__ void batch_copy( const CopyDesc* __restrict__ d )
+{
+    const CopyDesc& c = d[ blockIdx.y ];
+    const int64_t total = c.n * c.rows;
+    for ( int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x )
+    {
+        const int64_t r = i / c.n, k = i - r * c.n;
+        c.dst[ r * c.ldDst + k ] = c.src[ r * c.ldSrc + k ];
+    }
+}
+
 // Clears n floats (used instead of memset nodes inside captured graphs).
 __global__ void batch_fill_zero( float* const* __restrict__ ptrs, const int64_t* __restrict__ counts )
 {

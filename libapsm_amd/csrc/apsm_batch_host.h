@@ -136,6 +136,18 @@ int pmaxOfTp( int tp )
     }
 }
 
+// One block of one instance's train as batched-launch descriptors.
+struct TrainBlk
+{
+    apsm::EntryDesc e;
+    apsm::GramDesc g;
+    apsm::EvDesc v;
+    apsm::Seq2Desc s;
+    bool hasEv;
+    int umax, tp, newPer;
+    size_t lds;
+};
+
 // Host-side descriptor storage: typed arrays appended into one byte buffer; launches refer to
 // offsets that become device pointers once the buffer is uploaded.
 struct DescArena
@@ -184,17 +196,7 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
     std::vector<int64_t> zc;
     std::vector<apsm::StageDesc> stage;
     int64_t maxS = 0;
-    struct Blk
-    {
-        apsm::EntryDesc e;
-        apsm::GramDesc g;
-        apsm::EvDesc v;
-        apsm::Seq2Desc s;
-        bool hasEv;
-        int umax, tp, newPer;
-        size_t lds;
-    };
-    std::vector<std::vector<Blk>> blocks( n );
+    std::vector<std::vector<TrainBlk>> blocks( n );
     std::vector<apsm::ThetaDesc> theta( n );
     std::vector<apsm::DetDesc> det( n );
     for ( int i = 0; i < n; i++ )
@@ -225,14 +227,8 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
         const int newPer = (int)( q / 2 );
         const int64_t R2 = 2 * Ti;
         const int64_t nStepsTotal = R2 / q;
-        int64_t steps = std::max<int64_t>( 1, std::min<int64_t>( kBlockRowsMax / q, ( h->H - w2 ) / q + 1 ) );
-        while ( steps > 1 && ( w2 + steps * q ) / 2 > apsm::kActMax )
-            steps--;
-        if ( ( w2 + steps * q ) / 2 > apsm::kActMax )
-            throw ApsmError { APSM_ERR_UNSUPPORTED, "window too large for the blocked train path" };
-        h->Pbuf.ensure( (size_t)U * steps * newPer * 2 );
-        h->Gact.ensure( (size_t)apsm::triIdx( ( w2 + steps * q ) / 2 ) * 3 );
-        h->EvF.ensure( (size_t)U * steps * w2 );
+        const int64_t steps = blockedSteps( h );
+        reserveBlockedScratch( h, steps );
         const int tp = seq2Tp( Wc );
         const int nParts = std::min( apsm::kSeqThreads / Wc, ( Wc + tp - 1 ) / tp );
         if ( tp < 0 || nParts > apsm::kMaxParts || nParts * tp < Wc || nParts > pmaxOfTp( tp ) )
@@ -247,7 +243,7 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
             a0Next = sNext - std::min<int64_t>( w2, sNext );
             const int nNewC = (int)( nb * newPer );
             const int nActC = (int)( ( sLast - a0 ) / 2 );
-            Blk k {};
+            TrainBlk k {};
             k.umax = roundUpPow2Users( U );
             k.tp = tp;
             k.newPer = newPer;
@@ -405,7 +401,7 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
         {
             if ( r >= blocks[ i ].size() )
                 continue;
-            const Blk& k = blocks[ i ][ r ];
+            const TrainBlk& k = blocks[ i ][ r ];
             entry[ k.umax ].push_back( k.e );
             entryX[ k.umax ] = std::max( entryX[ k.umax ], k.e.a.nNewC );
             gram.push_back( k.g );

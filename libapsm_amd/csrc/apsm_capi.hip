@@ -376,6 +376,30 @@ void launchEntry( apsm_handle* h, const float* Z, const float* Zn, const float* 
     HIP_OR_THROW( hipGetLastError() );
 }
 
+// Projection steps per block of the blocked path: entering rows <= H - W + q (no active row
+// leaves the dictionary inside a block), active samples <= kActMax.
+int64_t blockedSteps( const apsm_handle* h )
+{
+    const int64_t q = h->p.sample_step;
+    const int w2 = h->w2;
+    int64_t steps = std::max<int64_t>( 1, std::min<int64_t>( kBlockRowsMax / q, ( h->H - w2 ) / q + 1 ) );
+    while ( steps > 1 && ( w2 + steps * q ) / 2 > apsm::kActMax )
+        steps--;
+    if ( ( w2 + steps * q ) / 2 > apsm::kActMax )
+        throw ApsmError { APSM_ERR_UNSUPPORTED, "window too large for the blocked train path" };
+    return steps;
+}
+
+// Work buffers of the blocked path for blocks of `steps` steps.
+void reserveBlockedScratch( apsm_handle* h, int64_t steps )
+{
+    const int64_t q = h->p.sample_step;
+    const int newPer = (int)( q / 2 );
+    h->Pbuf.ensure( (size_t)h->U * steps * newPer * 2 );
+    h->Gact.ensure( (size_t)apsm::triIdx( ( h->w2 + steps * q ) / 2 ) * 3 );
+    h->EvF.ensure( (size_t)h->U * steps * h->w2 );
+}
+
 // Runs the projection steps of one call in blocks (see apsm_train2.h).
 void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedOld, int64_t R2, int64_t rowBase, float wL, float wG,
                    float g2 )
@@ -384,16 +408,8 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
     const int w2 = h->w2, Wc = w2 / 2, U = h->U;
     const int newPer = (int)( q / 2 );
     const int64_t nStepsTotal = R2 / q;
-    // block size: entering rows <= H - W + q (no active row leaves the dictionary inside a
-    // block), active samples <= kActMax, step package <= prefetch registers
-    int64_t steps = std::max<int64_t>( 1, std::min<int64_t>( kBlockRowsMax / q, ( h->H - w2 ) / q + 1 ) );
-    while ( steps > 1 && ( w2 + steps * q ) / 2 > apsm::kActMax )
-        steps--;
-    if ( ( w2 + steps * q ) / 2 > apsm::kActMax )
-        throw ApsmError { APSM_ERR_UNSUPPORTED, "window too large for the blocked train path" };
-    h->Pbuf.ensure( (size_t)U * steps * newPer * 2 );
-    h->Gact.ensure( (size_t)apsm::triIdx( ( w2 + steps * q ) / 2 ) * 3 );
-    h->EvF.ensure( (size_t)U * steps * w2 );
+    const int64_t steps = blockedSteps( h );
+    reserveBlockedScratch( h, steps );
 
     const int tp = seq2Tp( Wc );
     if ( tp < 0 )
@@ -812,6 +828,7 @@ float elapsedMs( hipEvent_t a, hipEvent_t b )
 } // namespace
 
 #include "apsm_batch_host.h"
+#include "apsm_serve_host.h"
 
 extern "C" {
 
@@ -1617,6 +1634,228 @@ int apsm_batch_tuning( apsm_batch* B, int32_t* table, uint32_t cap_rows, uint32_
                 for ( int c = 0; c < 6; c++ )
                     table[ r * 6 + c ] = B->tuning[ r ][ c ];
         }
+    } );
+}
+
+// ---- serving ----
+int apsm_server_create( const apsm_server_config* cfg, apsm_server** out )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( cfg && out, "null argument" );
+        ARG_CHECK( cfg->frame_arena > 0 && cfg->frame_arena <= cfg->hbm_budget, "frame arena must be positive and within the budget" );
+        *out = nullptr;
+        apsm_server* S = new apsm_server();
+        try
+        {
+            S->cfg = *cfg;
+            S->epoch = std::chrono::steady_clock::now();
+            HIP_OR_THROW( hipStreamCreateWithFlags( &S->stream, hipStreamNonBlocking ) );
+            for ( int k = 0; k < kBatchSideStreams; k++ )
+            {
+                HIP_OR_THROW( hipStreamCreateWithFlags( &S->side[ k ], hipStreamNonBlocking ) );
+                HIP_OR_THROW( hipEventCreateWithFlags( &S->join[ k ], hipEventDisableTiming ) );
+            }
+            HIP_OR_THROW( hipEventCreateWithFlags( &S->fork, hipEventDisableTiming ) );
+            for ( ServeRound& R : S->rounds )
+            {
+                HIP_OR_THROW( hipEventCreateWithFlags( &R.detected, hipEventDisableTiming ) );
+                HIP_OR_THROW( hipEventCreateWithFlags( &R.done, hipEventDisableTiming ) );
+            }
+            int dev = 0;
+            HIP_OR_THROW( hipGetDevice( &dev ) );
+            HIP_OR_THROW( hipDeviceGetAttribute( &S->cus, hipDeviceAttributeMultiprocessorCount, dev ) );
+            S->arena.ensure( (size_t)cfg->frame_arena );
+            S->arenaAlloc.reset( (size_t)cfg->frame_arena );
+        }
+        catch ( ... )
+        {
+            apsm_server_destroy( S );
+            throw;
+        }
+        *out = S;
+    } );
+}
+
+void apsm_server_destroy( apsm_server* S )
+{
+    if ( !S )
+        return;
+    if ( S->stream )
+        (void)hipStreamSynchronize( S->stream );
+    // every frame is in exactly one of: waiting, training, a round's finishing list, finished
+    for ( ServeFrame* f : S->training )
+        delete f;
+    S->training.clear();
+    for ( ServeRound& R : S->rounds )
+    {
+        for ( ServeFrame* f : R.finishing )
+            delete f;
+        R.finishing.clear();
+        R.detectedFrames.clear();
+        if ( R.hostDesc )
+            (void)hipHostFree( R.hostDesc );
+        if ( R.detected )
+            (void)hipEventDestroy( R.detected );
+        if ( R.done )
+            (void)hipEventDestroy( R.done );
+    }
+    serverDestroyFrames( S );
+    for ( ServeLink& L : S->links )
+        apsm_destroy( L.h );
+    for ( int k = 0; k < kBatchSideStreams; k++ )
+    {
+        if ( S->side[ k ] )
+            (void)hipStreamDestroy( S->side[ k ] );
+        if ( S->join[ k ] )
+            (void)hipEventDestroy( S->join[ k ] );
+    }
+    if ( S->fork )
+        (void)hipEventDestroy( S->fork );
+    if ( S->stream )
+        (void)hipStreamDestroy( S->stream );
+    delete S;
+}
+
+int apsm_server_add_link( apsm_server* S, const apsm_params* p, uint32_t num_antennas, uint32_t num_users, uint32_t max_pilots,
+                          uint32_t* link )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( S && p && link, "null argument" );
+        ARG_CHECK( max_pilots >= 1, "max_pilots must be positive" );
+        apsm_handle* h = nullptr;
+        if ( apsm_create( p, num_antennas, num_users, &h ) != APSM_OK )
+            throw ApsmError { APSM_ERR_ARG, g_lastError };
+        try
+        {
+            checkServable( h );
+            // reserve every buffer a frame can need (serving never allocates)
+            const int64_t K = h->Hc + (int64_t)max_pilots;
+            h->buf[ 0 ].ensure( K, h->N, h->U );
+            h->buf[ 1 ].ensure( K, h->N, h->U );
+            reserveBlockedScratch( h, blockedSteps( h ) );
+            const size_t bytes = handleDeviceBytes( h );
+            if ( S->linkBytes + bytes + S->cfg.frame_arena > S->cfg.hbm_budget )
+                throw ApsmError { APSM_ERR_BUSY, "HBM budget exhausted: link not admitted" };
+            ServeLink L;
+            L.h = h;
+            L.bytes = bytes;
+            L.maxPilots = max_pilots;
+            S->links.push_back( L );
+            S->linkBytes += bytes;
+            *link = (uint32_t)( S->links.size() - 1 );
+        }
+        catch ( ... )
+        {
+            apsm_destroy( h );
+            throw;
+        }
+    } );
+}
+
+apsm_handle* apsm_server_link( apsm_server* S, uint32_t link ) { return ( S && link < S->links.size() ) ? S->links[ link ].h : nullptr; }
+
+int apsm_server_submit( apsm_server* S, const apsm_frame* f, uint64_t* ticket )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( S && f, "null argument" );
+        const uint64_t t = S->nextTicket++;
+        serverEnqueue( S, *f, serverNow( S ), t );
+        if ( ticket )
+            *ticket = t;
+    } );
+}
+
+int apsm_server_poll( apsm_server* S, int wait, apsm_frame_result* out, uint32_t cap, uint32_t* n )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( S && n, "null argument" );
+        do
+        {
+            serverRetire( S );
+            serverLaunchRound( S );
+        } while ( wait && ( !S->waiting.empty() || !S->training.empty() || !S->inFlight.empty() ) );
+        if ( wait )
+            HIP_OR_THROW( hipStreamSynchronize( S->stream ) ); // host-side ends of pageable copies
+        uint32_t k = 0;
+        while ( k < cap && !S->finished.empty() && out )
+        {
+            ServeFrame* f = S->finished.front();
+            fillResult( f, 0.0, &out[ k++ ] );
+            S->finished.erase( S->finished.begin() );
+            delete f;
+        }
+        *n = k;
+    } );
+}
+
+int apsm_server_replay( apsm_server* S, const apsm_frame* frames, uint32_t n, apsm_frame_result* results )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( S && ( n == 0 || ( frames && results ) ), "null argument" );
+        ARG_CHECK( S->waiting.empty() && S->training.empty() && S->inFlight.empty() && S->finished.empty(),
+                   "replay needs an idle server" );
+        for ( uint32_t i = 1; i < n; i++ )
+            ARG_CHECK( frames[ i ].arrival_s >= frames[ i - 1 ].arrival_s, "replay frames must be in arrival order" );
+        const uint64_t ticket0 = S->nextTicket;
+        S->nextTicket += n;
+        const double t0 = serverNow( S );
+        uint32_t next = 0, done = 0;
+        try
+        {
+            while ( done < n )
+            {
+                const double now = serverNow( S ) - t0;
+                while ( next < n && frames[ next ].arrival_s <= now )
+                {
+                    serverEnqueue( S, frames[ next ], t0 + frames[ next ].arrival_s, ticket0 + next );
+                    next++;
+                }
+                serverRetire( S );
+                serverLaunchRound( S );
+                for ( ServeFrame* f : S->finished )
+                {
+                    fillResult( f, t0, &results[ f->ticket - ticket0 ] );
+                    delete f;
+                    done++;
+                }
+                S->finished.clear();
+            }
+            HIP_OR_THROW( hipStreamSynchronize( S->stream ) ); // host-side ends of pageable copies
+        }
+        catch ( ... )
+        {
+            (void)hipStreamSynchronize( S->stream );
+            throw;
+        }
+    } );
+}
+
+int apsm_server_stats( apsm_server* S, uint64_t* rounds, uint64_t* max_frames_per_round, uint64_t* link_bytes )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( S, "null server" );
+        if ( rounds )
+            *rounds = S->roundsRun;
+        if ( max_frames_per_round )
+            *max_frames_per_round = S->maxBatch;
+        if ( link_bytes )
+            *link_bytes = S->linkBytes;
+    } );
+}
+
+int apsm_host_malloc( void** ptr, uint64_t bytes )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( ptr && bytes > 0, "bad allocation request" );
+        HIP_OR_THROW( hipHostMalloc( ptr, (size_t)bytes, hipHostMallocDefault ) );
+    } );
+}
+
+int apsm_host_free( void* ptr )
+{
+    return guarded( [ & ] {
+        if ( ptr )
+            HIP_OR_THROW( hipHostFree( ptr ) );
     } );
 }
 

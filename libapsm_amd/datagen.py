@@ -160,21 +160,55 @@ def batched_instances(count=512, seed=3):
     return out
 
 
-def serving_frames(count, seed=5, links=2048, rate_hz=1000.0):
-    """Requests of the serving workload: Poisson arrivals at rate_hz; each frame carries
-    14 OFDM symbols x (12 x U[1, 273]) subcarriers, 5 % of them pilots, on a link drawn
-    uniformly from `links` (8..32 antennas, QPSK or 16QAM per link)."""
+def serving_links(links=2048, seed=5):
+    """Per-link parameters of the serving workload: 8..32 antennas, QPSK or 16QAM."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    link_par = [dict(N=int(rng.integers(8, 33)), modulation=str(rng.choice(["QPSK", "QAM16"]))) for _ in range(links)]
+    return [dict(N=int(rng.integers(8, 33)), modulation=str(rng.choice(["QPSK", "QAM16"]))) for _ in range(links)]
+
+
+def serving_frames(count, seed=5, links=2048, rate_hz=1000.0, max_rb=273):
+    """Requests of the serving workload: Poisson arrivals at rate_hz; each frame carries
+    14 OFDM symbols x (12 x U[1, max_rb]) subcarriers of data plus 5 % as many pilots, on a
+    link drawn uniformly from `links` (8..32 antennas, QPSK or 16QAM per link)."""
+    link_par = serving_links(links, seed)
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
     t = 0.0
     frames = []
     for i in range(count):
         t += rng.exponential(1.0 / rate_hz)
-        n = 14 * 12 * int(rng.integers(1, 274))
+        n = 14 * 12 * int(rng.integers(1, max_rb + 1))
         pilots = max(1, int(round(0.05 * n)))
         link = int(rng.integers(0, links))
         frames.append(dict(arrival_s=t, link=link, samples=n, pilots=pilots, seed=5000000 + i, **link_par[link]))
     return frames
+
+
+def serving_payload(frame, U_tx=4, snr_db=15.0, user=0, link_seed_base=7000000):
+    """Samples of one serving frame: the link's Rayleigh channel (fixed per link, from
+    link_seed_base + link) carries U_tx superimposed users; symbols and noise come from the
+    frame's seed. Each antenna is scaled by its expected power (channel gains + noise), the
+    same scale for every frame of the link. Returns rx_pilots [N][P], tx_pilots [1][P] (the
+    detected user), rx_data [N][S], tx_data [S]."""
+    N, bps = frame["N"], BITS_PER_SYMBOL[frame["modulation"].upper()]
+    lr = np.random.Generator(np.random.PCG64(link_seed_base + frame["link"]))
+    H = (lr.standard_normal((N, U_tx)) + 1j * lr.standard_normal((N, U_tx))) / np.sqrt(2)
+    nvar = 10.0 ** (-snr_db / 10.0)
+    scale = 1.0 / np.sqrt(np.sum(np.abs(H) ** 2, axis=1) + nvar)
+    rng = np.random.Generator(np.random.PCG64(frame["seed"]))
+    P, S = frame["pilots"], frame["samples"]
+    pts = constellation(bps)
+    sym = pts[rng.integers(0, 1 << bps, size=(U_tx, P + S))]
+    noise = (rng.standard_normal((N, P + S)) + 1j * rng.standard_normal((N, P + S))) * np.sqrt(nvar / 2)
+    x = ((H @ sym + noise) * scale[:, None]).astype(np.complex64)
+    sym = sym.astype(np.complex64)
+    return dict(rx_pilots=np.ascontiguousarray(x[:, :P]), tx_pilots=np.ascontiguousarray(sym[user:user + 1, :P]),
+                rx_data=np.ascontiguousarray(x[:, P:]), tx_data=np.ascontiguousarray(sym[user, P:]), bps=bps)
+
+
+def serving_link_config(link_par, window=40, dict_size=1024, sigma=0.5, eB=0.01):
+    """Detector configuration of a serving link."""
+    return {"windowSize": window, "dictionarySize": dict_size, "eB": eB,
+            "gaussianKernelVariance": kernel_variance_for_sigma(sigma, link_par["N"])}
 
 
 def _main(argv=None):
