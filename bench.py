@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-instances measurement")
     ap.add_argument("--batched-instances", type=int, default=512)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision large-instance measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serving-frames", type=int, default=600)
+    ap.add_argument("--serving-links", type=int, default=2048)
     ap.add_argument("--mixed-samples", type=int, default=None, help="override the mixed workload's detect samples")
     return ap.parse_args()
 
@@ -291,6 +294,109 @@ def bench_mixed(steps, samples=None):
     return out
 
 
+def bench_serving(n_frames=600, n_links=2048, rates=(1000.0, 2000.0, 4000.0, 8000.0)):
+    """Serving loop: up to n_links links (8..32 antennas, QPSK or 16QAM, windowSize 40,
+    dictionary 1024) admitted by an HBM budget; frames of 14 OFDM symbols x 12 x U[1, 273]
+    subcarriers plus 5 % pilots on uniformly drawn links, Poisson arrivals swept over `rates`
+    (frames/s) and a saturation run (all frames at t = 0). Inputs and estimates live in pinned
+    host memory, so the latency includes the copies: latency = estimates in host memory -
+    arrival. The plain comparison serves the same frames one after another, each through a
+    handle of its own link (detect, then train; synchronous)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from libapsm_amd import ApsmError, ApsmNomaDetectorFactory, ApsmServer, Frame, PinnedArray, datagen, lib
+    from libapsm_amd._lib import check
+
+    links = datagen.serving_links(n_links, seed=5)
+    max_pilots = int(round(0.05 * 14 * 12 * 273)) + 1
+    budget, arena = 24 << 30, 4 << 30
+    t0 = time.perf_counter()
+    srv = ApsmServer(budget, arena)
+    admitted = 0
+    for lp in links:
+        try:
+            srv.add_link(datagen.serving_link_config(lp), lp["N"], 1, max_pilots)
+        except ApsmError as e:
+            if e.code != -5:
+                raise
+            break
+        admitted += 1
+    t_links = time.perf_counter() - t0
+    base = datagen.serving_frames(n_frames, seed=5, links=admitted, rate_hz=1.0)
+
+    def payload(f):
+        p = datagen.serving_payload(f)
+        out = {}
+        for k in ("rx_data", "rx_pilots", "tx_pilots"):
+            pa = PinnedArray(p[k].shape, np.complex64)
+            pa.array[...] = p[k]
+            out[k] = pa
+        out["est"] = PinnedArray((1, f["samples"]), np.complex64)
+        out["tx_data"] = p["tx_data"]
+        out["bps"] = p["bps"]
+        return out
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        pay = list(ex.map(payload, base))
+    t_gen = time.perf_counter() - t0
+    symbols = float(sum(f["samples"] for f in base))
+
+    def frames(scale):
+        return [Frame(f["link"], p["rx_data"].array, p["rx_pilots"].array, p["tx_pilots"].array,
+                      arrival_s=f["arrival_s"] * scale, est=p["est"].array) for f, p in zip(base, pay)]
+
+    srv.replay(frames(0.0)[:min(64, n_frames)])  # warm-up (code objects, first launches)
+    runs = []
+    for rate in list(rates) + [None]:
+        fr = frames(0.0 if rate is None else 1.0 / rate)
+        res = srv.replay(fr)
+        lat = np.array([r["latency_s"] for r in res]) * 1e3
+        span = max(r["output_s"] for r in res) - min(r["arrival_s"] for r in res)
+        runs.append({"offered_frames_per_s": rate, "p50_ms": float(np.percentile(lat, 50)),
+                     "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+                     "symbols_per_s": symbols / span, "frames_per_s": len(fr) / span})
+    errs = sum(datagen.symbol_errors(p["est"].array[0], p["tx_data"], p["bps"]) for p in pay)
+    st = srv.stats()
+
+    # plain: the same frames one after another, one handle per link, synchronous
+    used = sorted({f["link"] for f in base})
+    dets = {l: ApsmNomaDetectorFactory.build(dict(datagen.serving_link_config(links[l]), otype="APSM"), links[l]["N"])
+            for l in used}
+    for f, p in list(zip(base, pay))[:min(64, n_frames)]:
+        dets[f["link"]].detect(p["rx_data"].array)
+    check(lib().apsm_device_synchronize())
+    t0 = time.perf_counter()
+    lat_plain = []
+    for f, p in zip(base, pay):
+        t1 = time.perf_counter()
+        d = dets[f["link"]]
+        d.detect(p["rx_data"].array)
+        lat_plain.append(time.perf_counter() - t1)
+        d.train(p["rx_pilots"].array, p["tx_pilots"].array[0])
+    plain_s = time.perf_counter() - t0
+    sat = runs[-1]
+    return {
+        "workload": "serving-2048-links",
+        "links_admitted": admitted,
+        "link_bytes": st["link_bytes"],
+        "hbm_budget": budget,
+        "frame_arena": arena,
+        "frames": n_frames,
+        "detected_symbols": int(symbols),
+        "rates": runs,
+        "saturation_symbols_per_s": sat["symbols_per_s"],
+        "plain_symbols_per_s": symbols / plain_s,
+        "plain_detect_p50_ms": float(np.percentile(np.array(lat_plain) * 1e3, 50)),
+        "speedup_vs_plain": sat["symbols_per_s"] / (symbols / plain_s),
+        "ser": errs / symbols,
+        "rounds": st["rounds"],
+        "max_frames_per_round": st["max_frames_per_round"],
+        "links_s": t_links,
+        "datagen_s": t_gen,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -394,12 +500,14 @@ def main():
                       "apsm_ser_same_user": datagen.symbol_errors(est_c[0], tx_data[0], data["bps"]) / M}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, data, args.cpu_seconds)
-    if not args.no_batched or not args.no_mixed:
+    if not args.no_batched or not args.no_mixed or not args.no_serving:
         del d_rxt, d_txt, d_rxd, d_est, det
     if not args.no_batched:
         out["batched"] = bench_batched(args.batched_instances, max(1, args.steps))
     if not args.no_mixed:
         out["mixed_precision"] = bench_mixed(max(1, args.steps), args.mixed_samples)
+    if not args.no_serving:
+        out["serving"] = bench_serving(args.serving_frames, args.serving_links)
     print(json.dumps(out))
 
 

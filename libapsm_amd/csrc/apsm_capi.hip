@@ -1655,11 +1655,15 @@ int apsm_server_create( const apsm_server_config* cfg, apsm_server** out )
                 HIP_OR_THROW( hipStreamCreateWithFlags( &S->side[ k ], hipStreamNonBlocking ) );
                 HIP_OR_THROW( hipEventCreateWithFlags( &S->join[ k ], hipEventDisableTiming ) );
             }
+            HIP_OR_THROW( hipStreamCreateWithFlags( &S->dstream, hipStreamNonBlocking ) );
+            HIP_OR_THROW( hipStreamCreateWithFlags( &S->ustream, hipStreamNonBlocking ) );
             HIP_OR_THROW( hipEventCreateWithFlags( &S->fork, hipEventDisableTiming ) );
             for ( ServeRound& R : S->rounds )
-            {
-                HIP_OR_THROW( hipEventCreateWithFlags( &R.detected, hipEventDisableTiming ) );
                 HIP_OR_THROW( hipEventCreateWithFlags( &R.done, hipEventDisableTiming ) );
+            for ( ServeDetect& D : S->detects )
+            {
+                HIP_OR_THROW( hipEventCreateWithFlags( &D.uploaded, hipEventDisableTiming ) );
+                HIP_OR_THROW( hipEventCreateWithFlags( &D.detected, hipEventDisableTiming ) );
             }
             int dev = 0;
             HIP_OR_THROW( hipGetDevice( &dev ) );
@@ -1682,24 +1686,25 @@ void apsm_server_destroy( apsm_server* S )
         return;
     if ( S->stream )
         (void)hipStreamSynchronize( S->stream );
-    // every frame is in exactly one of: waiting, training, a round's finishing list, finished
-    for ( ServeFrame* f : S->training )
-        delete f;
-    S->training.clear();
+    if ( S->dstream )
+        (void)hipStreamSynchronize( S->dstream );
+    if ( S->ustream )
+        (void)hipStreamSynchronize( S->ustream );
+    serverDropFrames( S );
     for ( ServeRound& R : S->rounds )
     {
-        for ( ServeFrame* f : R.finishing )
-            delete f;
-        R.finishing.clear();
-        R.detectedFrames.clear();
-        if ( R.hostDesc )
-            (void)hipHostFree( R.hostDesc );
-        if ( R.detected )
-            (void)hipEventDestroy( R.detected );
+        R.desc.release();
         if ( R.done )
             (void)hipEventDestroy( R.done );
     }
-    serverDestroyFrames( S );
+    for ( ServeDetect& D : S->detects )
+    {
+        D.desc.release();
+        if ( D.uploaded )
+            (void)hipEventDestroy( D.uploaded );
+        if ( D.detected )
+            (void)hipEventDestroy( D.detected );
+    }
     for ( ServeLink& L : S->links )
         apsm_destroy( L.h );
     for ( int k = 0; k < kBatchSideStreams; k++ )
@@ -1713,6 +1718,10 @@ void apsm_server_destroy( apsm_server* S )
         (void)hipEventDestroy( S->fork );
     if ( S->stream )
         (void)hipStreamDestroy( S->stream );
+    if ( S->dstream )
+        (void)hipStreamDestroy( S->dstream );
+    if ( S->ustream )
+        (void)hipStreamDestroy( S->ustream );
     delete S;
 }
 
@@ -1770,12 +1779,10 @@ int apsm_server_poll( apsm_server* S, int wait, apsm_frame_result* out, uint32_t
     return guarded( [ & ] {
         ARG_CHECK( S && n, "null argument" );
         do
-        {
-            serverRetire( S );
-            serverLaunchRound( S );
-        } while ( wait && ( !S->waiting.empty() || !S->training.empty() || !S->inFlight.empty() ) );
+            serverProgress( S );
+        while ( wait && !serverIdle( S ) );
         if ( wait )
-            HIP_OR_THROW( hipStreamSynchronize( S->stream ) ); // host-side ends of pageable copies
+            HIP_OR_THROW( hipStreamSynchronize( S->dstream ) ); // host-side ends of pageable copies
         uint32_t k = 0;
         while ( k < cap && !S->finished.empty() && out )
         {
@@ -1792,8 +1799,7 @@ int apsm_server_replay( apsm_server* S, const apsm_frame* frames, uint32_t n, ap
 {
     return guarded( [ & ] {
         ARG_CHECK( S && ( n == 0 || ( frames && results ) ), "null argument" );
-        ARG_CHECK( S->waiting.empty() && S->training.empty() && S->inFlight.empty() && S->finished.empty(),
-                   "replay needs an idle server" );
+        ARG_CHECK( serverIdle( S ) && S->finished.empty(), "replay needs an idle server" );
         for ( uint32_t i = 1; i < n; i++ )
             ARG_CHECK( frames[ i ].arrival_s >= frames[ i - 1 ].arrival_s, "replay frames must be in arrival order" );
         const uint64_t ticket0 = S->nextTicket;
@@ -1810,8 +1816,7 @@ int apsm_server_replay( apsm_server* S, const apsm_frame* frames, uint32_t n, ap
                     serverEnqueue( S, frames[ next ], t0 + frames[ next ].arrival_s, ticket0 + next );
                     next++;
                 }
-                serverRetire( S );
-                serverLaunchRound( S );
+                serverProgress( S );
                 for ( ServeFrame* f : S->finished )
                 {
                     fillResult( f, t0, &results[ f->ticket - ticket0 ] );
@@ -1820,11 +1825,12 @@ int apsm_server_replay( apsm_server* S, const apsm_frame* frames, uint32_t n, ap
                 }
                 S->finished.clear();
             }
-            HIP_OR_THROW( hipStreamSynchronize( S->stream ) ); // host-side ends of pageable copies
+            HIP_OR_THROW( hipStreamSynchronize( S->dstream ) ); // host-side ends of pageable copies
         }
         catch ( ... )
         {
             (void)hipStreamSynchronize( S->stream );
+            (void)hipStreamSynchronize( S->dstream );
             throw;
         }
     } );
@@ -1835,7 +1841,7 @@ int apsm_server_stats( apsm_server* S, uint64_t* rounds, uint64_t* max_frames_pe
     return guarded( [ & ] {
         ARG_CHECK( S, "null server" );
         if ( rounds )
-            *rounds = S->roundsRun;
+            *rounds = S->roundsRun + S->detectsRun;
         if ( max_frames_per_round )
             *max_frames_per_round = S->maxBatch;
         if ( link_bytes )

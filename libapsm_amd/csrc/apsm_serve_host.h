@@ -4,21 +4,30 @@
 // stays on the device between frames, each admitted against an HBM budget with its buffers
 // reserved up front, so that serving never allocates. A frame of a link is: detect its data
 // samples with the link's current state, then train the link incrementally on the frame's
-// pilots. Frames are served by continuous batching in rounds on one stream: a round uploads
-// and detects every frame admitted since the previous round (one persistent batched detect
-// per kernel bucket), copies the estimates out, and then advances every link train in
-// progress by one block (the blocks trainBlocked forms, as batched launches). A frame admitted
-// while other frames train joins the next round; a link takes its next frame once the previous
-// frame's last block is enqueued (stream order does the rest). Frame buffers come from a
+// pilots. Frames are served by continuous batching on two streams:
+//   detect stream: every frame admitted since the previous detect batch is uploaded and
+//     detected in one batch (a persistent batched detect per kernel bucket), and its estimates
+//     are copied out; up to kServeDetectInFlight batches are queued;
+//   train stream: rounds; a round starts the trains of frames whose detect was enqueued
+//     (after an event wait on their detect batch), and advances every link train in progress
+//     by one block (the blocks trainBlocked forms, as batched launches); up to
+//     kServeRoundsInFlight rounds are queued.
+// A frame arriving while others are on the GPU is detected in the next detect batch, without
+// waiting for the trains, and its train joins the next round. A link takes its next frame once
+// the round with its previous frame's last block has completed. Frame buffers come from a
 // device arena of fixed size: a frame waits until the arena has room (first fit, in arrival
-// order). Up to kServeRoundsInFlight rounds are queued on the device while the host builds the
-// next one.
+// order).
 #pragma once
 
 #include <chrono>
 #include <deque>
 
 constexpr int kServeRoundsInFlight = 2;
+constexpr int kServeDetectInFlight = 4;
+// A detect batch closes at this many frames or bytes, so that the upload of the next batch,
+// this batch's detect and the trains overlap instead of running as one serial burst.
+constexpr size_t kServeBatchFrames = 32;
+constexpr size_t kServeBatchBytes = size_t( 64 ) << 20;
 
 // Descriptors of one incremental train call of one handle on the blocked path.
 struct CallPlan
@@ -288,7 +297,7 @@ struct ServeLink
     apsm_handle* h = nullptr;
     size_t bytes = 0;
     uint32_t maxPilots = 0;
-    bool busy = false; // a frame of this link is between admission and its last enqueued block
+    bool busy = false; // a frame of this link is admitted and its train has not completed
 };
 
 struct ServeFrame
@@ -301,19 +310,57 @@ struct ServeFrame
     float *dRxD = nullptr, *dRxP = nullptr, *dTxP = nullptr, *dEst = nullptr;
     CallPlan plan;
     size_t nextBlock = 0;
-    int64_t detectRound = -1, lastRound = -1;
+    int64_t detectBatch = -1, lastRound = -1;
     int32_t status = APSM_OK;
+};
+
+// Pinned host staging and device copy of one launch's descriptors.
+struct DescSlot
+{
+    DevBuf<char> dev;
+    char* host = nullptr;
+    size_t hostCap = 0;
+    const char* upload( const DescArena& a, hipStream_t st )
+    {
+        const size_t bytes = std::max<size_t>( a.bytes.size(), 256 );
+        if ( hostCap < bytes )
+        {
+            if ( host )
+                HIP_OR_THROW( hipHostFree( host ) );
+            host = nullptr;
+            hostCap = 0;
+            HIP_OR_THROW( hipHostMalloc( (void**)&host, bytes * 2, hipHostMallocDefault ) );
+            hostCap = bytes * 2;
+        }
+        std::memcpy( host, a.bytes.data(), a.bytes.size() );
+        dev.ensure( hostCap );
+        HIP_OR_THROW( hipMemcpyAsync( dev.ptr, host, a.bytes.size(), hipMemcpyHostToDevice, st ) );
+        return dev.ptr;
+    }
+    void release()
+    {
+        if ( host )
+            (void)hipHostFree( host );
+        host = nullptr;
+        hostCap = 0;
+        dev.release();
+    }
+};
+
+struct ServeDetect
+{
+    int64_t id = -1;
+    hipEvent_t uploaded = nullptr, detected = nullptr;
+    std::vector<ServeFrame*> frames; // frames without pilots finish here
+    DescSlot desc;
 };
 
 struct ServeRound
 {
     int64_t id = -1;
-    hipEvent_t detected = nullptr, done = nullptr;
-    bool detectSeen = false;
-    std::vector<ServeFrame*> detectedFrames, finishing;
-    DevBuf<char> desc;
-    char* hostDesc = nullptr; // pinned staging of the descriptors
-    size_t hostCap = 0;
+    hipEvent_t done = nullptr;
+    std::vector<ServeFrame*> finishing;
+    DescSlot desc;
 };
 
 struct apsm_server
@@ -323,19 +370,23 @@ struct apsm_server
     size_t linkBytes = 0;
     DevBuf<char> arena;
     RangeAlloc arenaAlloc;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr; // train rounds
+    hipStream_t dstream = nullptr; // detect batches
+    hipStream_t ustream = nullptr; // frame uploads (overlap the previous batch's detect)
     hipStream_t side[ kBatchSideStreams ] = {};
     hipEvent_t fork = nullptr, join[ kBatchSideStreams ] = {};
     ServeRound rounds[ kServeRoundsInFlight ];
-    std::deque<int64_t> inFlight; // round ids, oldest first
-    int64_t nextRound = 0;
+    ServeDetect detects[ kServeDetectInFlight ];
+    std::deque<int64_t> roundsInFlight, detectsInFlight; // ids, oldest first
+    int64_t nextRound = 0, nextDetect = 0;
     std::deque<ServeFrame*> waiting; // arrival order
+    std::vector<ServeFrame*> pendingTrain; // detected (enqueued), train not started
     std::vector<ServeFrame*> training; // trains in progress (blocks left)
     std::vector<ServeFrame*> finished; // completed, not yet reported by poll
     uint64_t nextTicket = 0;
     std::chrono::steady_clock::time_point epoch;
     int cus = 256;
-    uint64_t roundsRun = 0, maxBatch = 0;
+    uint64_t roundsRun = 0, detectsRun = 0, maxBatch = 0;
 };
 
 namespace {
@@ -356,62 +407,77 @@ size_t handleDeviceBytes( const apsm_handle* h )
     return b;
 }
 
+size_t alignUp256( size_t n ) { return ( n + 255 ) & ~size_t( 255 ); }
+
 size_t frameBytes( const ServeLink& L, const apsm_frame& f )
 {
     const size_t N = (size_t)L.h->N, U = (size_t)L.h->U;
-    auto al = []( size_t n ) { return ( n + 255 ) & ~size_t( 255 ); };
-    return al( N * f.n_data * 8 ) + al( N * f.n_pilots * 8 ) + al( U * f.n_pilots * 8 ) + al( U * f.n_data * 8 );
+    return alignUp256( N * f.n_data * 8 ) + alignUp256( N * f.n_pilots * 8 ) + alignUp256( U * f.n_pilots * 8 ) +
+        alignUp256( U * f.n_data * 8 );
 }
 
-// Retires finished rounds (in order): detect completion times, train completion times, arena.
+bool serverIdle( const apsm_server* S )
+{
+    return S->waiting.empty() && S->pendingTrain.empty() && S->training.empty() && S->roundsInFlight.empty() && S->detectsInFlight.empty();
+}
+
+void finishFrame( apsm_server* S, ServeFrame* f, double t )
+{
+    f->trained = t;
+    S->arenaAlloc.release( f->arenaOff, f->arenaBytes );
+    S->links[ f->f.link ].busy = false;
+    S->finished.push_back( f );
+}
+
+// Retires completed detect batches and train rounds (each in order).
 void serverRetire( apsm_server* S )
 {
-    while ( !S->inFlight.empty() )
+    while ( !S->detectsInFlight.empty() )
     {
-        ServeRound& R = S->rounds[ S->inFlight.front() % kServeRoundsInFlight ];
-        if ( !R.detectSeen )
+        ServeDetect& D = S->detects[ S->detectsInFlight.front() % kServeDetectInFlight ];
+        const hipError_t e = hipEventQuery( D.detected );
+        if ( e == hipErrorNotReady )
+            break;
+        HIP_OR_THROW( e );
+        const double t = serverNow( S );
+        for ( ServeFrame* f : D.frames )
         {
-            const hipError_t e = hipEventQuery( R.detected );
-            if ( e == hipErrorNotReady )
-                return;
-            HIP_OR_THROW( e );
-            const double t = serverNow( S );
-            for ( ServeFrame* f : R.detectedFrames )
-                f->output = t;
-            R.detectSeen = true;
+            f->output = t;
+            if ( f->f.n_pilots == 0 )
+                finishFrame( S, f, t );
         }
+        D.frames.clear();
+        S->detectsInFlight.pop_front();
+    }
+    while ( !S->roundsInFlight.empty() )
+    {
+        ServeRound& R = S->rounds[ S->roundsInFlight.front() % kServeRoundsInFlight ];
         const hipError_t e = hipEventQuery( R.done );
         if ( e == hipErrorNotReady )
-            return;
+            break;
         HIP_OR_THROW( e );
         const double t = serverNow( S );
         for ( ServeFrame* f : R.finishing )
-        {
-            f->trained = t;
-            S->arenaAlloc.release( f->arenaOff, f->arenaBytes );
-            S->finished.push_back( f );
-        }
-        R.detectedFrames.clear();
+            finishFrame( S, f, t );
         R.finishing.clear();
-        S->inFlight.pop_front();
+        S->roundsInFlight.pop_front();
     }
 }
 
-// Builds and launches one round if there is work and a round slot is free; returns whether it
-// launched one.
-bool serverLaunchRound( apsm_server* S )
+// Admits waiting frames and launches their detect batch on the detect stream; returns whether
+// it launched one.
+bool serverLaunchDetect( apsm_server* S )
 {
-    if ( (int)S->inFlight.size() >= kServeRoundsInFlight )
-        return false;
-    if ( S->waiting.empty() && S->training.empty() )
+    if ( S->waiting.empty() || (int)S->detectsInFlight.size() >= kServeDetectInFlight )
         return false;
     const double now = serverNow( S );
-
-    // admission: arrival order; a frame whose link is busy waits, a frame without arena room
-    // stops admission
+    // arrival order; a frame whose link is busy waits, a frame without arena room stops admission
     std::vector<ServeFrame*> admitted;
+    size_t batchBytes = 0;
     for ( auto it = S->waiting.begin(); it != S->waiting.end(); )
     {
+        if ( admitted.size() >= kServeBatchFrames || batchBytes >= kServeBatchBytes )
+            break;
         ServeFrame* f = *it;
         ServeLink& L = S->links[ f->f.link ];
         if ( L.busy )
@@ -425,31 +491,27 @@ bool serverLaunchRound( apsm_server* S )
         f->arenaOff = off;
         char* base = S->arena.ptr + off;
         const size_t N = (size_t)L.h->N, U = (size_t)L.h->U;
-        auto al = []( size_t n ) { return ( n + 255 ) & ~size_t( 255 ); };
         f->dRxD = (float*)base;
-        base += al( N * f->f.n_data * 8 );
+        base += alignUp256( N * f->f.n_data * 8 );
         f->dRxP = (float*)base;
-        base += al( N * f->f.n_pilots * 8 );
+        base += alignUp256( N * f->f.n_pilots * 8 );
         f->dTxP = (float*)base;
-        base += al( U * f->f.n_pilots * 8 );
+        base += alignUp256( U * f->f.n_pilots * 8 );
         f->dEst = (float*)base;
         f->admit = now;
         L.busy = true;
+        batchBytes += f->arenaBytes;
         admitted.push_back( f );
         it = S->waiting.erase( it );
     }
-    if ( admitted.empty() && S->training.empty() )
+    if ( admitted.empty() )
         return false;
 
-    const int64_t rid = S->nextRound++;
-    ServeRound& R = S->rounds[ rid % kServeRoundsInFlight ];
-    R.id = rid;
-    R.detectSeen = false;
-    R.detectedFrames = admitted;
-    R.finishing.clear();
+    const int64_t id = S->nextDetect++;
+    ServeDetect& D = S->detects[ id % kServeDetectInFlight ];
+    D.id = id;
+    D.frames = admitted;
     DescArena arena;
-
-    // detect buckets of the admitted frames (kernel, NMAX, UMAX)
     std::vector<apsm::DetDesc> det;
     std::map<std::array<int, 3>, std::vector<int>> dBuckets;
     for ( ServeFrame* f : admitted )
@@ -458,7 +520,7 @@ bool serverLaunchRound( apsm_server* S )
         det.push_back( detectDescOf( h, f->dRxD, f->f.n_data, f->dEst ) );
         dBuckets[ { detectKernelOf( h->p.detect_precision, (uint32_t)h->N ), nmaxOf( h->N ), roundUpPow2Users( h->U ) } ].push_back(
             (int)det.size() - 1 );
-        f->detectRound = rid;
+        f->detectBatch = id;
     }
     const size_t detOff = arena.add( det );
     struct DetL
@@ -477,35 +539,86 @@ bool serverLaunchRound( apsm_server* S )
                 items.push_back( apsm::WorkItem { i, (int)t } );
         detL.push_back( DetL { kernel, kv.first[ 1 ], kv.first[ 2 ], (int)items.size(), arena.add( items ) } );
     }
-
-    // train: new calls (compaction + staging + first block), then one block of every train
+    hipStream_t st = S->dstream, up = S->ustream;
+    const char* base = D.desc.upload( arena, up );
     for ( ServeFrame* f : admitted )
     {
-        ServeLink& L = S->links[ f->f.link ];
+        const apsm_handle* h = S->links[ f->f.link ].h;
+        const size_t N = (size_t)h->N, U = (size_t)h->U;
+        HIP_OR_THROW( hipMemcpyAsync( f->dRxD, f->f.rx_data, N * f->f.n_data * 8, hipMemcpyHostToDevice, up ) );
         if ( f->f.n_pilots > 0 )
         {
-            planTrainCall( L.h, f->dRxP, f->dTxP, f->f.n_pilots, f->plan );
-            f->nextBlock = 0;
-            S->training.push_back( f );
+            HIP_OR_THROW( hipMemcpyAsync( f->dRxP, f->f.rx_pilots, N * f->f.n_pilots * 8, hipMemcpyHostToDevice, up ) );
+            HIP_OR_THROW( hipMemcpyAsync( f->dTxP, f->f.tx_pilots, U * f->f.n_pilots * 8, hipMemcpyHostToDevice, up ) );
         }
+    }
+    HIP_OR_THROW( hipEventRecord( D.uploaded, up ) );
+    HIP_OR_THROW( hipStreamWaitEvent( st, D.uploaded, 0 ) );
+    const apsm::DetDesc* dd = (const apsm::DetDesc*)( base + detOff );
+    for ( const DetL& d : detL )
+    {
+        const int grid = std::max( 1, std::min( d.nItems, S->cus * 4 ) );
+        const apsm::WorkItem* it = (const apsm::WorkItem*)( base + d.itemOff );
+        if ( d.kernel < 0 )
+            launchBatchDetectAny( st, d.nmax, d.umax, apsm::kDetThreads, dd, it, d.nItems, grid );
         else
-        {
-            L.busy = false;
-            f->lastRound = rid;
-            R.finishing.push_back( f );
-        }
+            HIP_OR_THROW( apsm::launchBatchDetectMfma( d.kernel, d.nmax, d.umax, st, dd, it, d.nItems, grid ) );
+    }
+    for ( ServeFrame* f : admitted )
+    {
+        const size_t U = (size_t)S->links[ f->f.link ].h->U;
+        HIP_OR_THROW( hipMemcpyAsync( f->f.est, f->dEst, U * f->f.n_data * 8, hipMemcpyDeviceToHost, st ) );
+    }
+    HIP_OR_THROW( hipEventRecord( D.detected, st ) );
+    for ( ServeFrame* f : admitted )
+        if ( f->f.n_pilots > 0 )
+            S->pendingTrain.push_back( f );
+    S->detectsInFlight.push_back( id );
+    S->detectsRun++;
+    S->maxBatch = std::max<uint64_t>( S->maxBatch, admitted.size() );
+    return true;
+}
+
+// Builds and launches one train round on the train stream; returns whether it launched one.
+bool serverLaunchTrainRound( apsm_server* S )
+{
+    if ( (int)S->roundsInFlight.size() >= kServeRoundsInFlight || ( S->pendingTrain.empty() && S->training.empty() ) )
+        return false;
+    const int64_t rid = S->nextRound++;
+    ServeRound& R = S->rounds[ rid % kServeRoundsInFlight ];
+    R.id = rid;
+    R.finishing.clear();
+    hipStream_t st = S->stream;
+
+    // trains starting now: after their detect batch (it reads the state they change)
+    std::vector<int64_t> waitOn;
+    for ( ServeFrame* f : S->pendingTrain )
+    {
+        planTrainCall( S->links[ f->f.link ].h, f->dRxP, f->dTxP, f->f.n_pilots, f->plan );
+        f->nextBlock = 0;
+        if ( std::find( waitOn.begin(), waitOn.end(), f->detectBatch ) == waitOn.end() )
+            waitOn.push_back( f->detectBatch );
+    }
+    for ( int64_t b : waitOn )
+    {
+        // a batch still in flight: wait for its event (a retired one has completed)
+        if ( std::find( S->detectsInFlight.begin(), S->detectsInFlight.end(), b ) != S->detectsInFlight.end() )
+            HIP_OR_THROW( hipStreamWaitEvent( st, S->detects[ b % kServeDetectInFlight ].detected, 0 ) );
     }
     std::vector<apsm::CopyDesc> copies;
     std::vector<apsm::StageDesc> stage;
     int64_t copyMax = 0, maxS = 0;
-    for ( ServeFrame* f : admitted )
-        if ( f->f.n_pilots > 0 )
-        {
-            copies.insert( copies.end(), f->plan.copies.begin(), f->plan.copies.end() );
-            copyMax = std::max( copyMax, f->plan.copyMax );
-            stage.push_back( f->plan.stage );
-            maxS = std::max<int64_t>( maxS, f->f.n_pilots );
-        }
+    for ( ServeFrame* f : S->pendingTrain )
+    {
+        copies.insert( copies.end(), f->plan.copies.begin(), f->plan.copies.end() );
+        copyMax = std::max( copyMax, f->plan.copyMax );
+        stage.push_back( f->plan.stage );
+        maxS = std::max<int64_t>( maxS, f->f.n_pilots );
+        S->training.push_back( f );
+    }
+    S->pendingTrain.clear();
+
+    DescArena arena;
     std::map<int, std::vector<apsm::EntryDesc>> entry;
     std::map<int, int> entryX;
     std::vector<apsm::GramDesc> gram;
@@ -537,7 +650,6 @@ bool serverLaunchRound( apsm_server* S )
         {
             theta.push_back( f->plan.theta );
             maxU = std::max( maxU, f->plan.theta.U );
-            S->links[ f->f.link ].busy = false;
             f->lastRound = rid;
             R.finishing.push_back( f );
             it = S->training.erase( it );
@@ -545,7 +657,6 @@ bool serverLaunchRound( apsm_server* S )
         else
             ++it;
     }
-    S->maxBatch = std::max<uint64_t>( S->maxBatch, admitted.size() + S->training.size() + R.finishing.size() );
     const size_t copyOff = arena.add( copies ), stageOff = arena.add( stage ), gramOff = arena.add( gram ), evOff = arena.add( evict ),
                  thetaOff = arena.add( theta );
     std::vector<std::pair<int, size_t>> entryOff;
@@ -561,54 +672,8 @@ bool serverLaunchRound( apsm_server* S )
     for ( auto& kv : seq )
         sl.push_back( SeqL { arena.add( kv.second ), (int)kv.second.size(), kv.first.first, kv.first.second, seqDim[ kv.first ].first,
                              seqDim[ kv.first ].second } );
+    const char* base = R.desc.upload( arena, st );
 
-    // upload the descriptors (pinned staging of this round slot; its previous round retired)
-    const size_t bytes = std::max<size_t>( arena.bytes.size(), 256 );
-    if ( R.hostCap < bytes )
-    {
-        if ( R.hostDesc )
-            HIP_OR_THROW( hipHostFree( R.hostDesc ) );
-        R.hostDesc = nullptr;
-        R.hostCap = 0;
-        HIP_OR_THROW( hipHostMalloc( (void**)&R.hostDesc, bytes * 2, hipHostMallocDefault ) );
-        R.hostCap = bytes * 2;
-    }
-    std::memcpy( R.hostDesc, arena.bytes.data(), arena.bytes.size() );
-    R.desc.ensure( R.hostCap );
-    const char* base = R.desc.ptr;
-    hipStream_t st = S->stream;
-    HIP_OR_THROW( hipMemcpyAsync( R.desc.ptr, R.hostDesc, arena.bytes.size(), hipMemcpyHostToDevice, st ) );
-
-    // frame inputs, detect, estimates out
-    for ( ServeFrame* f : admitted )
-    {
-        const apsm_handle* h = S->links[ f->f.link ].h;
-        const size_t N = (size_t)h->N, U = (size_t)h->U;
-        HIP_OR_THROW( hipMemcpyAsync( f->dRxD, f->f.rx_data, N * f->f.n_data * 8, hipMemcpyHostToDevice, st ) );
-        if ( f->f.n_pilots > 0 )
-        {
-            HIP_OR_THROW( hipMemcpyAsync( f->dRxP, f->f.rx_pilots, N * f->f.n_pilots * 8, hipMemcpyHostToDevice, st ) );
-            HIP_OR_THROW( hipMemcpyAsync( f->dTxP, f->f.tx_pilots, U * f->f.n_pilots * 8, hipMemcpyHostToDevice, st ) );
-        }
-    }
-    const apsm::DetDesc* dd = (const apsm::DetDesc*)( base + detOff );
-    for ( const DetL& d : detL )
-    {
-        const int grid = std::max( 1, std::min( d.nItems, S->cus * 4 ) );
-        const apsm::WorkItem* it = (const apsm::WorkItem*)( base + d.itemOff );
-        if ( d.kernel < 0 )
-            launchBatchDetectAny( st, d.nmax, d.umax, apsm::kDetThreads, dd, it, d.nItems, grid );
-        else
-            HIP_OR_THROW( apsm::launchBatchDetectMfma( d.kernel, d.nmax, d.umax, st, dd, it, d.nItems, grid ) );
-    }
-    for ( ServeFrame* f : admitted )
-    {
-        const size_t U = (size_t)S->links[ f->f.link ].h->U;
-        HIP_OR_THROW( hipMemcpyAsync( f->f.est, f->dEst, U * f->f.n_data * 8, hipMemcpyDeviceToHost, st ) );
-    }
-    HIP_OR_THROW( hipEventRecord( R.detected, st ) );
-
-    // train launches
     if ( !copies.empty() )
     {
         hipLaunchKernelGGL( apsm::batch_copy, dim3( std::min( 64, gridFor( copyMax, 256 ) ), (unsigned)copies.size() ), dim3( 256 ), 0, st,
@@ -665,9 +730,17 @@ bool serverLaunchRound( apsm_server* S )
         HIP_OR_THROW( hipGetLastError() );
     }
     HIP_OR_THROW( hipEventRecord( R.done, st ) );
-    S->inFlight.push_back( rid );
+    S->roundsInFlight.push_back( rid );
     S->roundsRun++;
     return true;
+}
+
+// One step of the serving loop: retire, then launch what can be launched.
+void serverProgress( apsm_server* S )
+{
+    serverRetire( S );
+    serverLaunchDetect( S );
+    serverLaunchTrainRound( S );
 }
 
 void serverEnqueue( apsm_server* S, const apsm_frame& f, double arrival, uint64_t ticket )
@@ -690,16 +763,6 @@ void serverEnqueue( apsm_server* S, const apsm_frame& f, double arrival, uint64_
     S->waiting.push_back( sf );
 }
 
-void serverDestroyFrames( apsm_server* S )
-{
-    for ( ServeFrame* f : S->waiting )
-        delete f;
-    for ( ServeFrame* f : S->finished )
-        delete f;
-    S->waiting.clear();
-    S->finished.clear();
-}
-
 void fillResult( const ServeFrame* f, double t0, apsm_frame_result* r )
 {
     r->status = f->status;
@@ -708,8 +771,39 @@ void fillResult( const ServeFrame* f, double t0, apsm_frame_result* r )
     r->admit_s = f->admit - t0;
     r->output_s = f->output - t0;
     r->trained_s = f->trained - t0;
-    r->detect_round = f->detectRound;
+    r->detect_round = f->detectBatch;
     r->last_round = f->lastRound;
+}
+
+// Deletes every frame the server still holds (each is in exactly one list).
+void serverDropFrames( apsm_server* S )
+{
+    std::vector<ServeFrame*> all( S->waiting.begin(), S->waiting.end() );
+    all.insert( all.end(), S->pendingTrain.begin(), S->pendingTrain.end() );
+    all.insert( all.end(), S->training.begin(), S->training.end() );
+    all.insert( all.end(), S->finished.begin(), S->finished.end() );
+    for ( ServeRound& R : S->rounds )
+    {
+        all.insert( all.end(), R.finishing.begin(), R.finishing.end() );
+        R.finishing.clear();
+    }
+    for ( ServeDetect& D : S->detects )
+    {
+        // frames of a detect batch are also in pendingTrain, training, a round or finished,
+        // except frames without pilots, which finish at the batch
+        for ( ServeFrame* f : D.frames )
+            if ( f->f.n_pilots == 0 )
+                all.push_back( f );
+        D.frames.clear();
+    }
+    std::sort( all.begin(), all.end() );
+    all.erase( std::unique( all.begin(), all.end() ), all.end() );
+    for ( ServeFrame* f : all )
+        delete f;
+    S->waiting.clear();
+    S->pendingTrain.clear();
+    S->training.clear();
+    S->finished.clear();
 }
 
 } // namespace
