@@ -21,13 +21,6 @@ struct StageDesc
     int N, U;
 };
 
-struct EntryDesc
-{
-    const float *Z, *Zn, *C, *thF;
-    float* P;
-    EntryArgs a;
-};
-
 struct GramDesc
 {
     const float *Z, *Zn;
@@ -62,15 +55,6 @@ __global__ void batch_stage( const StageDesc* __restrict__ d )
 {
     const StageDesc& s = d[ blockIdx.y ];
     stagePilotsBody( s.rx, s.tx, nullptr, s.N, s.U, s.S, s.Z, s.Zn, s.Y, s.C, s.ldYC, (int64_t)blockIdx.x * blockDim.x + threadIdx.x );
-}
-
-template <int UMAX>
-__global__ void __launch_bounds__( kEntryThreads ) batch_entry( const EntryDesc* __restrict__ d )
-{
-    const EntryDesc& e = d[ blockIdx.y ];
-    if ( (int)blockIdx.x >= e.a.nNewC )
-        return;
-    entryFrozenBody<UMAX>( e.Z, e.Zn, e.C, e.thF, e.P, e.a, blockIdx.x );
 }
 
 __global__ void batch_gram( const GramDesc* __restrict__ d )

@@ -80,25 +80,6 @@ void launchBatchDetectAny( hipStream_t s, int nmax, int umax, int threads, const
     }
 }
 
-template <int UMAX>
-void launchBatchEntry( hipStream_t s, const apsm::EntryDesc* d, int n, int gridX )
-{
-    hipLaunchKernelGGL( ( apsm::batch_entry<UMAX> ), dim3( gridX, n ), dim3( apsm::kEntryThreads ), 0, s, d );
-    HIP_OR_THROW( hipGetLastError() );
-}
-
-void launchBatchEntryAny( hipStream_t s, int umax, const apsm::EntryDesc* d, int n, int gridX )
-{
-    switch ( umax )
-    {
-    case 1: launchBatchEntry<1>( s, d, n, gridX ); break;
-    case 2: launchBatchEntry<2>( s, d, n, gridX ); break;
-    case 4: launchBatchEntry<4>( s, d, n, gridX ); break;
-    case 8: launchBatchEntry<8>( s, d, n, gridX ); break;
-    default: launchBatchEntry<16>( s, d, n, gridX ); break;
-    }
-}
-
 template <int TP, int NEWPER, int PMAX>
 void launchBatchSeq2( hipStream_t s, const apsm::Seq2Desc* d, int n, int gridX, size_t lds )
 {
@@ -139,7 +120,8 @@ int pmaxOfTp( int tp )
 // One block of one instance's train as batched-launch descriptors.
 struct TrainBlk
 {
-    apsm::EntryDesc e;
+    apsm::EntryMfmaDesc e;
+    int nmax;
     apsm::GramDesc g;
     apsm::EvDesc v;
     apsm::Seq2Desc s;
@@ -162,6 +144,49 @@ struct DescArena
         return off;
     }
 };
+
+// The entry share of a round: one chunked launch per (NMAX, UMAX) bucket, then one combine over
+// every descriptor of the round (all buckets' descriptors are contiguous at combOff).
+struct EntryLaunch
+{
+    std::map<std::pair<int, int>, std::vector<apsm::EntryMfmaDesc>> buckets;
+    std::map<std::pair<int, int>, int> maxChunks;
+    int maxCombine = 0;
+    void add( const TrainBlk& k )
+    {
+        const auto key = std::make_pair( k.nmax, k.umax );
+        buckets[ key ].push_back( k.e );
+        maxChunks[ key ] = std::max( maxChunks[ key ], k.e.nChunks );
+        maxCombine = std::max( maxCombine, k.e.a.U * k.e.a.nNewC );
+    }
+    // descriptors into the arena: per bucket, then all of them again for the combine
+    size_t upload( DescArena& arena, std::vector<std::pair<std::pair<int, int>, size_t>>& offs, int& nComb ) const
+    {
+        std::vector<apsm::EntryMfmaDesc> all;
+        for ( auto& kv : buckets )
+        {
+            offs.push_back( { kv.first, arena.add( kv.second ) } );
+            all.insert( all.end(), kv.second.begin(), kv.second.end() );
+        }
+        nComb = (int)all.size();
+        return arena.add( all );
+    }
+};
+
+void launchRoundEntry( hipStream_t s, const EntryLaunch& el, const char* base, const std::vector<std::pair<std::pair<int, int>, size_t>>& offs,
+                       size_t combOff, int nComb )
+{
+    for ( auto& o : offs )
+    {
+        const int gx = el.maxChunks.at( o.first );
+        if ( gx > 0 )
+            HIP_OR_THROW( apsm::launchBatchEntryMfma( o.first.first, o.first.second, s, (const apsm::EntryMfmaDesc*)( base + o.second ),
+                                                      (int)el.buckets.at( o.first ).size(), gx ) );
+    }
+    if ( nComb > 0 )
+        HIP_OR_THROW( apsm::launchBatchEntryCombine( s, (const apsm::EntryMfmaDesc*)( base + combOff ), nComb, ( el.maxCombine + 255 ) / 256 ) );
+}
+
 
 void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT, const uint32_t* T, const float* const* rxD, const uint32_t* M,
                 float* const* est, bool useGraph )
@@ -262,11 +287,14 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
             ea.wL = wL;
             ea.wG = wG;
             ea.g2 = g2;
+            k.nmax = nmaxOf( N );
             k.e.Z = b.Z.ptr;
             k.e.Zn = b.Zn.ptr;
             k.e.C = b.C.ptr;
             k.e.thF = h->thOld.ptr;
             k.e.P = h->Pbuf.ptr;
+            k.e.part = h->Ppart.ptr;
+            apsm::entryChunks( ea, k.e.jBeg, k.e.nChunks );
             apsm::GramArgs& ga = k.g.a;
             ga.N = N;
             ga.L = L;
@@ -389,8 +417,7 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
         maxBlocks = std::max( maxBlocks, v.size() );
     for ( size_t r = 0; r < maxBlocks; r++ )
     {
-        std::map<int, std::vector<apsm::EntryDesc>> entry;
-        std::map<int, int> entryX;
+        EntryLaunch entry;
         std::vector<apsm::GramDesc> gram;
         int64_t maxPairs = 0;
         std::vector<apsm::EvDesc> evict;
@@ -402,8 +429,7 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
             if ( r >= blocks[ i ].size() )
                 continue;
             const TrainBlk& k = blocks[ i ][ r ];
-            entry[ k.umax ].push_back( k.e );
-            entryX[ k.umax ] = std::max( entryX[ k.umax ], k.e.a.nNewC );
+            entry.add( k );
             gram.push_back( k.g );
             maxPairs = std::max<int64_t>( maxPairs, apsm::triIdx( k.g.a.nA ) );
             if ( k.hasEv )
@@ -417,13 +443,11 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
             dim.first = std::max( dim.first, k.s.a.U );
             dim.second = std::max( dim.second, k.lds );
         }
-        for ( auto& kv : entry )
         {
-            const size_t off = arena.add( kv.second );
-            const int cnt = (int)kv.second.size(), umax = kv.first, gx = entryX[ kv.first ];
-            pend.push_back( { 0, [ = ]( const char* base ) {
-                                 launchBatchEntryAny( B->stream, umax, (const apsm::EntryDesc*)( base + off ), cnt, gx );
-                             } } );
+            std::vector<std::pair<std::pair<int, int>, size_t>> offs;
+            int nComb = 0;
+            const size_t combOff = entry.upload( arena, offs, nComb );
+            pend.push_back( { 0, [ = ]( const char* base ) { launchRoundEntry( B->stream, entry, base, offs, combOff, nComb ); } } );
         }
         if ( !gram.empty() )
         {

@@ -130,6 +130,10 @@ struct StreamBuf
 
 constexpr float kLog2e = 1.4426950408889634f;
 
+// Template buckets of the antenna count and of a user group.
+int nmaxOf( int N ) { return N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64; }
+int roundUpPow2Users( int U ) { return U <= 1 ? 1 : U <= 2 ? 2 : U <= 4 ? 4 : U <= 8 ? 8 : 16; }
+
 // APSM_PRECISION_FP32 runs the cross terms on the matrix cores from this many antennas on
 // (below, the vector-ALU kernel is at least as fast).
 constexpr uint32_t kMatrixFp32MinAntennas = 1;
@@ -255,7 +259,7 @@ struct apsm_handle
     DevBuf<float> F, Gram, thOld, theta; // thOld: evicted-row linear share (per-call path) or thF (blocked path)
     bool blocked = false; // blocked train path (register-resident window Gram)
     int64_t a0Next = 0; // blocked path: frozen boundary after the last step (absolute real row)
-    DevBuf<float> Pbuf, Gact, EvF; // blocked path work buffers
+    DevBuf<float> Pbuf, Gact, EvF, Ppart; // blocked path work buffers (Ppart: entry-share chunk partials)
     DevBuf<float> GramG, FG; // norm-constraint path: Gaussian window Gram and Gaussian part of f
     DevBuf<double> Qn; // norm-constraint path: c^T K c per user
     DevBuf<float> rx, tx, est; // staging for the host entry points
@@ -368,14 +372,6 @@ int seq2Tp( int Wc )
     return -1;
 }
 
-template <int UMAX>
-void launchEntry( apsm_handle* h, const float* Z, const float* Zn, const float* C, const float* thF, float* P, const apsm::EntryArgs& ea )
-{
-    hipLaunchKernelGGL( ( apsm::apsm_entry_frozen<UMAX> ), dim3( ea.nNewC ), dim3( apsm::kEntryThreads ), 0, h->stream, Z, Zn, C, thF, P,
-                        ea );
-    HIP_OR_THROW( hipGetLastError() );
-}
-
 // Projection steps per block of the blocked path: entering rows <= H - W + q (no active row
 // leaves the dictionary inside a block), active samples <= kActMax.
 int64_t blockedSteps( const apsm_handle* h )
@@ -396,6 +392,7 @@ void reserveBlockedScratch( apsm_handle* h, int64_t steps )
     const int64_t q = h->p.sample_step;
     const int newPer = (int)( q / 2 );
     h->Pbuf.ensure( (size_t)h->U * steps * newPer * 2 );
+    h->Ppart.ensure( (size_t)h->U * ( h->Hc / apsm::kEntryChunk + 2 ) * steps * newPer * 2 );
     h->Gact.ensure( (size_t)apsm::triIdx( ( h->w2 + steps * q ) / 2 ) * 3 );
     h->EvF.ensure( (size_t)h->U * steps * h->w2 );
 }
@@ -443,6 +440,9 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
         ea.wL = wL;
         ea.wG = wG;
         ea.g2 = g2;
+        int64_t jBeg = 0;
+        int nChunks = 0;
+        apsm::entryChunks( ea, jBeg, nChunks );
         for ( int u0 = 0; u0 < U; u0 += 16 )
         {
             // users in groups of 16 share one launch
@@ -451,16 +451,11 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
             const float* Cg = b.C.ptr + (size_t)u0 * ld;
             const float* thg = h->thOld.ptr + (size_t)u0 * h->L;
             float* Pg = h->Pbuf.ptr + (size_t)u0 * nNewC * 2;
-            if ( eu.U <= 1 )
-                launchEntry<1>( h, b.Z.ptr, b.Zn.ptr, Cg, thg, Pg, eu );
-            else if ( eu.U <= 2 )
-                launchEntry<2>( h, b.Z.ptr, b.Zn.ptr, Cg, thg, Pg, eu );
-            else if ( eu.U <= 4 )
-                launchEntry<4>( h, b.Z.ptr, b.Zn.ptr, Cg, thg, Pg, eu );
-            else if ( eu.U <= 8 )
-                launchEntry<8>( h, b.Z.ptr, b.Zn.ptr, Cg, thg, Pg, eu );
-            else
-                launchEntry<16>( h, b.Z.ptr, b.Zn.ptr, Cg, thg, Pg, eu );
+            float* partG = h->Ppart.ptr + (size_t)u0 * nChunks * nNewC * 2;
+            if ( nChunks > 0 )
+                HIP_OR_THROW( apsm::launchEntryMfma( nmaxOf( h->N ), roundUpPow2Users( eu.U ), h->stream, b.Z.ptr, b.Zn.ptr, Cg, partG, eu,
+                                                     jBeg, nChunks ) );
+            HIP_OR_THROW( apsm::launchEntryCombine( h->stream, b.Z.ptr, thg, partG, Pg, eu, nChunks ) );
         }
 
         apsm::GramArgs ga {};
@@ -770,8 +765,6 @@ int detectKernelOf( int32_t precision, uint32_t N )
     }
 }
 
-int nmaxOf( int N ) { return N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64; }
-int roundUpPow2Users( int U ) { return U <= 1 ? 1 : U <= 2 ? 2 : U <= 4 ? 4 : U <= 8 ? 8 : 16; }
 
 void detectDev( apsm_handle* h, const float* d_rx, int64_t M, float* d_est )
 {
@@ -969,6 +962,7 @@ void apsm_destroy( apsm_handle* h )
     h->FG.release();
     h->Qn.release();
     h->Pbuf.release();
+    h->Ppart.release();
     h->Gact.release();
     h->EvF.release();
     h->thOld.release();

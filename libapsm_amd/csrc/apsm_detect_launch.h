@@ -16,6 +16,44 @@ constexpr int PREC_F16 = 2;
 constexpr int kMfmaThreads = 256; // 4 waves, 32 samples each
 constexpr int kMfmaSamples = 128; // samples per workgroup tile
 
+struct EntryArgs
+{
+    int N, L, U, q;
+    int64_t rowBase; // absolute real row of buffer row 0
+    int64_t newC0; // local complex index of the first entering sample
+    int64_t sFirst; // absolute s of the block's first step
+    int64_t H;
+    int64_t a0; // absolute frozen boundary (real row, even)
+    int64_t ldYC;
+    int nNewC;
+    float wL, wG, g2;
+};
+
+// Frozen-dictionary share of a block's entering pilots on the matrix cores: the frozen range
+// is cut into chunks of kEntryChunk entries (one workgroup each), then combined.
+constexpr int kEntryChunk = 128;
+
+// First frozen entry of the block's first entering sample and the number of chunks.
+inline void entryChunks( const EntryArgs& a, int64_t& jBeg, int& nChunks )
+{
+    const int64_t rt = a.rowBase + 2 * a.newC0;
+    const int64_t se = rt < a.sFirst ? a.sFirst : a.sFirst + a.q * ( ( rt - a.sFirst ) / a.q + 1 );
+    const int64_t lo = se > a.H ? se - a.H : 0;
+    jBeg = ( lo - a.rowBase ) / 2;
+    const int64_t j1 = ( a.a0 - a.rowBase ) / 2;
+    nChunks = j1 > jBeg ? (int)( ( j1 - jBeg + kEntryChunk - 1 ) / kEntryChunk ) : 0;
+}
+
+// One instance of a batched entry share: buffers, partial sums [U][nChunks][nNewC][2], output P.
+struct EntryMfmaDesc
+{
+    const float *Z, *Zn, *C, *thF;
+    float *P, *part;
+    EntryArgs a;
+    int64_t jBeg;
+    int nChunks;
+};
+
 // One instance of a batched detect: its data samples, dictionary, coefficients and weights.
 struct DetDesc
 {
@@ -38,6 +76,16 @@ struct WorkItem
 hipError_t launchDetectMfma( int prec, int nmax, int umax, hipStream_t s, const float* rx, int64_t M, const float* Z,
                              const float* Zn, const float* C, int64_t ldC, const float* theta, int64_t G, int U, float wL,
                              float wG, float g2, float* est, int N );
+
+// Entry share of one block (nChunks > 0 chunk workgroups), then the combine of the chunks and the
+// linear share into P [U][nNewC][2]; batched variants over n descriptors (gridX = most chunks,
+// resp. enough threads for the largest U * nNewC).
+hipError_t launchEntryMfma( int nmax, int umax, hipStream_t s, const float* Z, const float* Zn, const float* C, float* part,
+                            const EntryArgs& a, int64_t jBeg, int nChunks );
+hipError_t launchEntryCombine( hipStream_t s, const float* Z, const float* thF, const float* part, float* P, const EntryArgs& a,
+                               int nChunks );
+hipError_t launchBatchEntryMfma( int nmax, int umax, hipStream_t s, const EntryMfmaDesc* d, int n, int gridX );
+hipError_t launchBatchEntryCombine( hipStream_t s, const EntryMfmaDesc* d, int n, int gridX );
 
 // Persistent batched detect: `grid` workgroups walk nItems work items of tiles of kMfmaSamples.
 hipError_t launchBatchDetectMfma( int prec, int nmax, int umax, hipStream_t s, const DetDesc* d, const WorkItem* items,

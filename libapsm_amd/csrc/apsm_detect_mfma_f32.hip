@@ -1,11 +1,37 @@
 // Matrix-core detect kernels with f32 cross-term inputs, every antenna and user bound.
 #include "apsm_detect_mfma.h"
+#include "apsm_entry_mfma.h"
 
 namespace apsm {
 
 APSM_MFMA_ENTRY( mfmaEntryF32 )
 {
     return launchMfmaPrec<PREC_F32>( nmax, umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
+}
+
+hipError_t launchEntryMfma( int nmax, int umax, hipStream_t s, const float* Z, const float* Zn, const float* C, float* part,
+                            const EntryArgs& a, int64_t jBeg, int nChunks )
+{
+    return launchEntryAnyN( nmax, umax, false, s, Z, Zn, C, part, a, jBeg, nChunks, nullptr, 0, 0 );
+}
+
+hipError_t launchEntryCombine( hipStream_t s, const float* Z, const float* thF, const float* part, float* P, const EntryArgs& a,
+                               int nChunks )
+{
+    const int n = a.U * a.nNewC;
+    hipLaunchKernelGGL( apsm_entry_combine, dim3( ( n + 255 ) / 256 ), dim3( 256 ), 0, s, Z, thF, part, P, a, nChunks );
+    return hipGetLastError();
+}
+
+hipError_t launchBatchEntryMfma( int nmax, int umax, hipStream_t s, const EntryMfmaDesc* d, int n, int gridX )
+{
+    return launchEntryAnyN( nmax, umax, true, s, nullptr, nullptr, nullptr, nullptr, EntryArgs {}, 0, 0, d, n, gridX );
+}
+
+hipError_t launchBatchEntryCombine( hipStream_t s, const EntryMfmaDesc* d, int n, int gridX )
+{
+    hipLaunchKernelGGL( batch_entry_combine, dim3( gridX, n ), dim3( 256 ), 0, s, d );
+    return hipGetLastError();
 }
 
 } // namespace apsm

@@ -159,11 +159,14 @@ void planTrainCall( apsm_handle* h, const float* d_rx, const float* d_tx, int64_
         ea.wL = wL;
         ea.wG = wG;
         ea.g2 = g2;
+        k.nmax = nmaxOf( N );
         k.e.Z = nb.Z.ptr;
         k.e.Zn = nb.Zn.ptr;
         k.e.C = nb.C.ptr;
         k.e.thF = h->thOld.ptr;
         k.e.P = h->Pbuf.ptr;
+        k.e.part = h->Ppart.ptr;
+        apsm::entryChunks( ea, k.e.jBeg, k.e.nChunks );
         apsm::GramArgs& ga = k.g.a;
         ga.N = N;
         ga.L = L;
@@ -401,7 +404,8 @@ size_t handleDeviceBytes( const apsm_handle* h )
     size_t b = 0;
     for ( const StreamBuf& s : h->buf )
         b += ( s.Z.cap + s.Zn.cap + s.Y.cap + s.C.cap ) * sizeof( float );
-    b += ( h->F.cap + h->Gram.cap + h->thOld.cap + h->theta.cap + h->Pbuf.cap + h->Gact.cap + h->EvF.cap + h->GramG.cap + h->FG.cap ) *
+    b += ( h->F.cap + h->Gram.cap + h->thOld.cap + h->theta.cap + h->Pbuf.cap + h->Gact.cap + h->EvF.cap + h->Ppart.cap + h->GramG.cap +
+           h->FG.cap ) *
         sizeof( float );
     b += h->Qn.cap * sizeof( double );
     return b;
@@ -619,8 +623,7 @@ bool serverLaunchTrainRound( apsm_server* S )
     S->pendingTrain.clear();
 
     DescArena arena;
-    std::map<int, std::vector<apsm::EntryDesc>> entry;
-    std::map<int, int> entryX;
+    EntryLaunch entry;
     std::vector<apsm::GramDesc> gram;
     int64_t maxPairs = 0, maxEv = 0;
     std::vector<apsm::EvDesc> evict;
@@ -632,8 +635,7 @@ bool serverLaunchTrainRound( apsm_server* S )
     {
         ServeFrame* f = *it;
         const TrainBlk& k = f->plan.blocks[ f->nextBlock++ ];
-        entry[ k.umax ].push_back( k.e );
-        entryX[ k.umax ] = std::max( entryX[ k.umax ], k.e.a.nNewC );
+        entry.add( k );
         gram.push_back( k.g );
         maxPairs = std::max<int64_t>( maxPairs, apsm::triIdx( k.g.a.nA ) );
         if ( k.hasEv )
@@ -659,9 +661,9 @@ bool serverLaunchTrainRound( apsm_server* S )
     }
     const size_t copyOff = arena.add( copies ), stageOff = arena.add( stage ), gramOff = arena.add( gram ), evOff = arena.add( evict ),
                  thetaOff = arena.add( theta );
-    std::vector<std::pair<int, size_t>> entryOff;
-    for ( auto& kv : entry )
-        entryOff.push_back( { kv.first, arena.add( kv.second ) } );
+    std::vector<std::pair<std::pair<int, int>, size_t>> entryOffs;
+    int nComb = 0;
+    const size_t combOff = entry.upload( arena, entryOffs, nComb );
     struct SeqL
     {
         size_t off;
@@ -686,8 +688,7 @@ bool serverLaunchTrainRound( apsm_server* S )
                             (const apsm::StageDesc*)( base + stageOff ) );
         HIP_OR_THROW( hipGetLastError() );
     }
-    for ( auto& eo : entryOff )
-        launchBatchEntryAny( st, eo.first, (const apsm::EntryDesc*)( base + eo.second ), (int)entry[ eo.first ].size(), entryX[ eo.first ] );
+    launchRoundEntry( st, entry, base, entryOffs, combOff, nComb );
     if ( !gram.empty() )
     {
         hipLaunchKernelGGL( apsm::batch_gram, dim3( gridFor( maxPairs, 256 ), (unsigned)gram.size() ), dim3( 256 ), 0, st,

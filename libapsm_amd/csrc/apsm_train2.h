@@ -3,7 +3,7 @@
 // For one block (steps s_first .. s_last, entering rows [s_prev, s_last)):
 //   a0   = s_first - W(s_first): rows below a0 take no further projection in this block
 //          (frozen); rows in [a0, s_last) are the block's active rows.
-//   apsm_entry_frozen  (whole GPU) for every entering complex sample t and every user:
+//   apsm_entry_mfma    (whole GPU) for every entering complex sample t and every user:
 //          P_t = wL*thF.x_t + wG*sum_{j frozen, in the dictionary when t enters} c_j K(j,t),
 //          with thF = theta_0 + sum_{j < a0} c_j x_j.
 //   apsm_active_gram   kappa(t,j) = wL x_t.x_j + wG K(t,j) for every pair of active complex
@@ -20,11 +20,11 @@
 #pragma once
 
 #include "apsm_device.h"
+#include "apsm_detect_launch.h"
 
 namespace apsm {
 
 constexpr int kActMax = 512; // active complex samples per block (host guarantees)
-constexpr int kEntryThreads = 256;
 constexpr int kMaxParts = 32; // t-slot parts of the register Gram (1024 / Wc for Wc >= 32)
 constexpr int kRedF = 2048; // per-part f partial slots (>= PMAX * Wc of every instantiation)
 // Thread roles in the sequencer's entering phase: [0, w2) reduce f (w2 <= 320), [kColBase,
@@ -48,98 +48,6 @@ __device__ __forceinline__ void pairDots( const float* __restrict__ x, const flo
         sp = fmaf( xr, zi, sp );
         sp = fmaf( -xi, zr, sp );
     }
-}
-
-struct EntryArgs
-{
-    int N, L, U, q;
-    int64_t rowBase; // absolute real row of buffer row 0
-    int64_t newC0; // local complex index of the first entering sample
-    int64_t sFirst; // absolute s of the block's first step
-    int64_t H;
-    int64_t a0; // absolute frozen boundary (real row, even)
-    int64_t ldYC;
-    int nNewC;
-    float wL, wG, g2;
-};
-
-// One workgroup per entering complex sample i.
-template <int UMAX>
-__device__ __forceinline__ void entryFrozenBody( const float* __restrict__ Z, const float* __restrict__ Zn, const float* __restrict__ C,
-                                                 const float* __restrict__ thF, float* __restrict__ P, const EntryArgs& a, const int i )
-{
-    __shared__ float sx[ 128 ];
-    __shared__ float sred[ kEntryThreads / kWave ][ 2 * UMAX ];
-    const int tid = threadIdx.x;
-    const int N = a.N, L = a.L;
-    const int64_t tc = a.newC0 + i;
-    for ( int d = tid; d < L; d += kEntryThreads )
-        sx[ d ] = Z[ tc * L + d ];
-    __syncthreads();
-    const float xn = Zn[ tc ];
-    // entry step of this sample and the dictionary lower bound at that step
-    const int64_t rt = a.rowBase + 2 * tc;
-    const int64_t se = rt < a.sFirst ? a.sFirst : a.sFirst + a.q * ( ( rt - a.sFirst ) / a.q + 1 );
-    const int64_t lo = se > a.H ? se - a.H : 0;
-    const int64_t j0 = ( lo - a.rowBase ) / 2, j1 = ( a.a0 - a.rowBase ) / 2;
-    float pa[ UMAX ], pb[ UMAX ];
-#pragma unroll
-    for ( int u = 0; u < UMAX; u++ )
-        pa[ u ] = pb[ u ] = 0.f;
-    for ( int64_t j = j0 + tid; j < j1; j += kEntryThreads )
-    {
-        float p, sp;
-        pairDots( sx, Z + j * L, N, p, sp );
-        const float nn = xn + Zn[ j ];
-        const float E0 = gaussFromDist( nn - 2.f * p, a.g2 );
-        const float E1 = gaussFromDist( nn - 2.f * sp, a.g2 );
-        const float E2 = gaussFromDist( nn + 2.f * sp, a.g2 );
-#pragma unroll
-        for ( int u = 0; u < UMAX; u++ )
-        {
-            if ( u < a.U )
-            {
-                const float ca = C[ u * a.ldYC + 2 * j ], cb = C[ u * a.ldYC + 2 * j + 1 ];
-                pa[ u ] = fmaf( ca, E0, fmaf( cb, E1, pa[ u ] ) );
-                pb[ u ] = fmaf( ca, E2, fmaf( cb, E0, pb[ u ] ) );
-            }
-        }
-    }
-    const int lane = tid & ( kWave - 1 ), wave = tid / kWave;
-#pragma unroll
-    for ( int u = 0; u < UMAX; u++ )
-    {
-        float va = waveSum( pa[ u ] ), vb = waveSum( pb[ u ] );
-        if ( lane == 0 )
-        {
-            sred[ wave ][ 2 * u ] = va;
-            sred[ wave ][ 2 * u + 1 ] = vb;
-        }
-    }
-    __syncthreads();
-    if ( tid < 2 * a.U && tid < 2 * UMAX )
-    {
-        const int u = tid >> 1, part = tid & 1;
-        float g = 0.f;
-        for ( int w = 0; w < kEntryThreads / kWave; w++ )
-            g += sred[ w ][ tid ];
-        const float* th = thF + u * L;
-        float lin = 0.f;
-        for ( int n = 0; n < N; n++ )
-        {
-            const float xr = sx[ n ], xi = sx[ N + n ];
-            lin += part == 0 ? ( th[ n ] * xr + th[ N + n ] * xi ) : ( th[ n ] * xi - th[ N + n ] * xr );
-        }
-        P[ ( (int64_t)u * a.nNewC + i ) * 2 + part ] = a.wL * lin + a.wG * g;
-    }
-}
-
-template <int UMAX>
-__global__ void __launch_bounds__( kEntryThreads ) apsm_entry_frozen( const float* __restrict__ Z, const float* __restrict__ Zn,
-                                                                     const float* __restrict__ C, const float* __restrict__ thF,
-                                                                     float* __restrict__ P, const EntryArgs a )
-{
-    entryFrozenBody<UMAX>( Z, Zn, C, thF, P, a, blockIdx.x );
 }
 
 struct GramArgs
