@@ -21,13 +21,6 @@ struct StageDesc
     int N, U;
 };
 
-struct GramDesc
-{
-    const float *Z, *Zn;
-    float* Gact;
-    GramArgs a;
-};
-
 struct EvDesc
 {
     const float *Z, *Zn, *C;
@@ -55,12 +48,6 @@ __global__ void batch_stage( const StageDesc* __restrict__ d )
 {
     const StageDesc& s = d[ blockIdx.y ];
     stagePilotsBody( s.rx, s.tx, nullptr, s.N, s.U, s.S, s.Z, s.Zn, s.Y, s.C, s.ldYC, (int64_t)blockIdx.x * blockDim.x + threadIdx.x );
-}
-
-__global__ void batch_gram( const GramDesc* __restrict__ d )
-{
-    const GramDesc& g = d[ blockIdx.y ];
-    activeGramBody( g.Z, g.Zn, g.Gact, g.a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x );
 }
 
 __global__ void batch_evict( const EvDesc* __restrict__ d )

@@ -418,8 +418,8 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
     for ( size_t r = 0; r < maxBlocks; r++ )
     {
         EntryLaunch entry;
-        std::vector<apsm::GramDesc> gram;
-        int64_t maxPairs = 0;
+        std::map<int, std::vector<apsm::GramDesc>> gram; // by NMAX
+        std::map<int, int64_t> gramBlocksMax;
         std::vector<apsm::EvDesc> evict;
         int64_t maxEv = 0;
         std::map<std::pair<int, int>, std::vector<apsm::Seq2Desc>> seq;
@@ -430,8 +430,8 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
                 continue;
             const TrainBlk& k = blocks[ i ][ r ];
             entry.add( k );
-            gram.push_back( k.g );
-            maxPairs = std::max<int64_t>( maxPairs, apsm::triIdx( k.g.a.nA ) );
+            gram[ k.nmax ].push_back( k.g );
+            gramBlocksMax[ k.nmax ] = std::max( gramBlocksMax[ k.nmax ], apsm::gramBlocks( k.g.a.nA ) );
             if ( k.hasEv )
             {
                 evict.push_back( k.v );
@@ -449,14 +449,12 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
             const size_t combOff = entry.upload( arena, offs, nComb );
             pend.push_back( { 0, [ = ]( const char* base ) { launchRoundEntry( B->stream, entry, base, offs, combOff, nComb ); } } );
         }
-        if ( !gram.empty() )
+        for ( auto& kv : gram )
         {
-            const size_t off = arena.add( gram );
-            const int cnt = (int)gram.size(), gx = gridFor( maxPairs, 256 );
+            const size_t off = arena.add( kv.second );
+            const int cnt = (int)kv.second.size(), nmax = kv.first, gx = (int)( ( gramBlocksMax[ kv.first ] + 3 ) / 4 );
             pend.push_back( { 0, [ = ]( const char* base ) {
-                                 hipLaunchKernelGGL( apsm::batch_gram, dim3( gx, cnt ), dim3( 256 ), 0, B->stream,
-                                                     (const apsm::GramDesc*)( base + off ) );
-                                 HIP_OR_THROW( hipGetLastError() );
+                                 HIP_OR_THROW( apsm::launchBatchGramMfma( nmax, B->stream, (const apsm::GramDesc*)( base + off ), cnt, gx ) );
                              } } );
         }
         if ( !evict.empty() )

@@ -466,10 +466,7 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
         ga.wL = wL;
         ga.wG = wG;
         ga.g2 = g2;
-        const int64_t pairs = apsm::triIdx( nActC );
-        hipLaunchKernelGGL( apsm::apsm_active_gram, dim3( gridFor( pairs, 256 ) ), dim3( 256 ), 0, h->stream, b.Z.ptr, b.Zn.ptr,
-                            h->Gact.ptr, ga );
-        HIP_OR_THROW( hipGetLastError() );
+        HIP_OR_THROW( apsm::launchGramMfma( nmaxOf( h->N ), h->stream, b.Z.ptr, b.Zn.ptr, h->Gact.ptr, ga ) );
 
         const int64_t loLast = sLast > h->H ? sLast - h->H : 0, loPrev = sPrev > h->H ? sPrev - h->H : 0;
         const int hasEv = loLast > loPrev ? 1 : 0;

@@ -54,6 +54,31 @@ struct EntryMfmaDesc
     int nChunks;
 };
 
+// Gram among a block's active samples: Gact[tri(i) + j][3] for active samples j <= i (relative
+// to c0), kappa = wL x.z + wG K in the three real forms A = kappa(a_i,a_j) = kappa(b_i,b_j),
+// B = kappa(a_i,b_j), C = kappa(b_i,a_j); 32 x 32 sample blocks of the lower triangle on the
+// matrix cores, four per workgroup.
+struct GramArgs
+{
+    int N, L;
+    int64_t c0; // local complex index of the first active sample
+    int64_t nA; // active complex samples
+    float wL, wG, g2;
+};
+
+struct GramDesc
+{
+    const float *Z, *Zn;
+    float* Gact;
+    GramArgs a;
+};
+
+__host__ __device__ inline int64_t gramBlocks( int64_t nA )
+{
+    const int64_t nb = ( nA + 31 ) / 32;
+    return nb * ( nb + 1 ) / 2;
+}
+
 // One instance of a batched detect: its data samples, dictionary, coefficients and weights.
 struct DetDesc
 {
@@ -86,6 +111,9 @@ hipError_t launchEntryCombine( hipStream_t s, const float* Z, const float* thF, 
                                int nChunks );
 hipError_t launchBatchEntryMfma( int nmax, int umax, hipStream_t s, const EntryMfmaDesc* d, int n, int gridX );
 hipError_t launchBatchEntryCombine( hipStream_t s, const EntryMfmaDesc* d, int n, int gridX );
+
+hipError_t launchGramMfma( int nmax, hipStream_t s, const float* Z, const float* Zn, float* Gact, const GramArgs& a );
+hipError_t launchBatchGramMfma( int nmax, hipStream_t s, const GramDesc* d, int n, int gridX );
 
 // Persistent batched detect: `grid` workgroups walk nItems work items of tiles of kMfmaSamples.
 hipError_t launchBatchDetectMfma( int prec, int nmax, int umax, hipStream_t s, const DetDesc* d, const WorkItem* items,

@@ -1,6 +1,7 @@
 // Matrix-core detect kernels with f32 cross-term inputs, every antenna and user bound.
 #include "apsm_detect_mfma.h"
 #include "apsm_entry_mfma.h"
+#include "apsm_gram_mfma.h"
 
 namespace apsm {
 
@@ -32,6 +33,18 @@ hipError_t launchBatchEntryCombine( hipStream_t s, const EntryMfmaDesc* d, int n
 {
     hipLaunchKernelGGL( batch_entry_combine, dim3( gridX, n ), dim3( 256 ), 0, s, d );
     return hipGetLastError();
+}
+
+hipError_t launchGramMfma( int nmax, hipStream_t s, const float* Z, const float* Zn, float* Gact, const GramArgs& a )
+{
+    if ( a.nA <= 0 )
+        return hipSuccess;
+    return launchGramAnyN( nmax, false, s, Z, Zn, Gact, a, nullptr, 0, 0 );
+}
+
+hipError_t launchBatchGramMfma( int nmax, hipStream_t s, const GramDesc* d, int n, int gridX )
+{
+    return launchGramAnyN( nmax, true, s, nullptr, nullptr, nullptr, GramArgs {}, d, n, gridX );
 }
 
 } // namespace apsm

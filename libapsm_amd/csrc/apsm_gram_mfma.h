@@ -1,0 +1,124 @@
+// Gram among the active samples of a train block on the matrix cores (FP32 inputs and
+// accumulation). One wave computes a 32 x 32 block (rows i of block bi as the B operand, held
+// in registers; rows j of block bj <= bi as the A operand, loaded from the stream buffer) of the
+// two cross terms p = a_i.a_j and s = a_i.b_j with v_mfma_f32_32x32x2f32, and writes the lower
+// triangle's kappa values; a workgroup takes four blocks.
+#pragma once
+
+#include "apsm_detect_mfma.h"
+
+namespace apsm {
+
+template <int NMAX>
+__device__ __forceinline__ void gramBlockMfma( const float* __restrict__ Z, const float* __restrict__ Zn, float* __restrict__ Gact,
+                                               const GramArgs& a, const int64_t blk )
+{
+    constexpr int K = 2 * NMAX;
+    const int N = a.N, L = a.L;
+    const int lane = threadIdx.x & 63;
+    const int col = lane & 31, h = lane >> 5;
+    // block (bi, bj), bj <= bi, from the linear lower-triangle index
+    int64_t bi = (int64_t)( ( sqrt( 8.0 * (double)blk + 1.0 ) - 1.0 ) * 0.5 );
+    while ( ( bi + 1 ) * ( bi + 2 ) / 2 <= blk )
+        bi++;
+    while ( bi * ( bi + 1 ) / 2 > blk )
+        bi--;
+    const int64_t bj = blk - bi * ( bi + 1 ) / 2;
+    const int64_t i = bi * 32 + col; // this lane's row (B operand column)
+    const int64_t jr = bj * 32 + col; // the row this lane supplies as A operand
+    const bool vi = i < a.nA, vj = jr < a.nA;
+    const float* xi = Z + ( a.c0 + i ) * L;
+    const float* zj = Z + ( a.c0 + jr ) * L;
+
+    mf32x16 accP = {}, accS = {};
+#pragma unroll
+    for ( int t = 0; t < K / 2; t++ )
+    {
+        const int k = 2 * t + h;
+        const int ant = k < NMAX ? k : k - NMAX;
+        float xv = 0.f, zp = 0.f, zs = 0.f;
+        if ( ant < N )
+        {
+            if ( vi )
+                xv = xi[ k < NMAX ? ant : N + ant ];
+            if ( vj )
+            {
+                const float zr = zj[ ant ], zim = zj[ N + ant ];
+                zp = k < NMAX ? zr : zim; // Zp row = [Re z | Im z]
+                zs = k < NMAX ? zim : -zr; // Zs row = [Im z | -Re z]
+            }
+        }
+        accP = __builtin_amdgcn_mfma_f32_32x32x2f32( zp, xv, accP, 0, 0, 0 );
+        accS = __builtin_amdgcn_mfma_f32_32x32x2f32( zs, xv, accS, 0, 0, 0 );
+        if ( ( t & 7 ) == 7 )
+            __builtin_amdgcn_sched_barrier( 0 );
+    }
+    if ( !vi )
+        return;
+    const float g2 = a.g2, k2 = 2.f * g2;
+    const float qi = -g2 * Zn[ a.c0 + i ];
+    const int64_t rowBase = i * ( i + 1 ) / 2;
+#pragma unroll
+    for ( int v = 0; v < 16; v++ )
+    {
+        const int64_t j = bj * 32 + 8 * ( v / 4 ) + 4 * h + ( v % 4 );
+        if ( j > i || j >= a.nA )
+            continue;
+        const float q = qi - g2 * Zn[ a.c0 + j ];
+        const float p = accP[ v ], s = accS[ v ];
+        const float E0 = __builtin_amdgcn_exp2f( fminf( fmaf( k2, p, q ), 0.f ) );
+        const float E1 = __builtin_amdgcn_exp2f( fminf( fmaf( k2, s, q ), 0.f ) );
+        const float E2 = __builtin_amdgcn_exp2f( fminf( fmaf( -k2, s, q ), 0.f ) );
+        float* o = Gact + ( rowBase + j ) * 3;
+        o[ 0 ] = a.wL * p + a.wG * E0;
+        o[ 1 ] = a.wL * s + a.wG * E1;
+        o[ 2 ] = -a.wL * s + a.wG * E2;
+    }
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__( kMfmaThreads ) apsm_gram_mfma( const float* __restrict__ Z, const float* __restrict__ Zn,
+                                                                 float* __restrict__ Gact, const GramArgs a )
+{
+    const int64_t blk = (int64_t)blockIdx.x * ( kMfmaThreads / 64 ) + ( threadIdx.x >> 6 );
+    if ( blk < gramBlocks( a.nA ) )
+        gramBlockMfma<NMAX>( Z, Zn, Gact, a, blk );
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__( kMfmaThreads ) batch_gram_mfma( const GramDesc* __restrict__ d )
+{
+    const GramDesc& g = d[ blockIdx.y ];
+    const int64_t blk = (int64_t)blockIdx.x * ( kMfmaThreads / 64 ) + ( threadIdx.x >> 6 );
+    if ( blk < gramBlocks( g.a.nA ) )
+        gramBlockMfma<NMAX>( g.Z, g.Zn, g.Gact, g.a, blk );
+}
+
+template <int NMAX>
+hipError_t launchGramN( bool batch, hipStream_t s, const float* Z, const float* Zn, float* Gact, const GramArgs& a, const GramDesc* d,
+                        int n, int gridX )
+{
+    constexpr int perWg = kMfmaThreads / 64;
+    if ( batch )
+        hipLaunchKernelGGL( ( batch_gram_mfma<NMAX> ), dim3( gridX, n ), dim3( kMfmaThreads ), 0, s, d );
+    else
+        hipLaunchKernelGGL( ( apsm_gram_mfma<NMAX> ), dim3( (unsigned)( ( gramBlocks( a.nA ) + perWg - 1 ) / perWg ) ), dim3( kMfmaThreads ), 0, s,
+                            Z, Zn, Gact, a );
+    return hipGetLastError();
+}
+
+inline hipError_t launchGramAnyN( int nmax, bool batch, hipStream_t s, const float* Z, const float* Zn, float* Gact, const GramArgs& a,
+                                  const GramDesc* d, int n, int gridX )
+{
+    switch ( nmax )
+    {
+    case 4: return launchGramN<4>( batch, s, Z, Zn, Gact, a, d, n, gridX );
+    case 8: return launchGramN<8>( batch, s, Z, Zn, Gact, a, d, n, gridX );
+    case 16: return launchGramN<16>( batch, s, Z, Zn, Gact, a, d, n, gridX );
+    case 32: return launchGramN<32>( batch, s, Z, Zn, Gact, a, d, n, gridX );
+    case 64: return launchGramN<64>( batch, s, Z, Zn, Gact, a, d, n, gridX );
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace apsm

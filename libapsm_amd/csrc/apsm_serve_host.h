@@ -624,8 +624,9 @@ bool serverLaunchTrainRound( apsm_server* S )
 
     DescArena arena;
     EntryLaunch entry;
-    std::vector<apsm::GramDesc> gram;
-    int64_t maxPairs = 0, maxEv = 0;
+    std::map<int, std::vector<apsm::GramDesc>> gram; // by NMAX
+    std::map<int, int64_t> gramBlocksMax;
+    int64_t maxEv = 0;
     std::vector<apsm::EvDesc> evict;
     std::map<std::pair<int, int>, std::vector<apsm::Seq2Desc>> seq;
     std::map<std::pair<int, int>, std::pair<int, size_t>> seqDim;
@@ -636,8 +637,8 @@ bool serverLaunchTrainRound( apsm_server* S )
         ServeFrame* f = *it;
         const TrainBlk& k = f->plan.blocks[ f->nextBlock++ ];
         entry.add( k );
-        gram.push_back( k.g );
-        maxPairs = std::max<int64_t>( maxPairs, apsm::triIdx( k.g.a.nA ) );
+        gram[ k.nmax ].push_back( k.g );
+        gramBlocksMax[ k.nmax ] = std::max( gramBlocksMax[ k.nmax ], apsm::gramBlocks( k.g.a.nA ) );
         if ( k.hasEv )
         {
             evict.push_back( k.v );
@@ -659,8 +660,10 @@ bool serverLaunchTrainRound( apsm_server* S )
         else
             ++it;
     }
-    const size_t copyOff = arena.add( copies ), stageOff = arena.add( stage ), gramOff = arena.add( gram ), evOff = arena.add( evict ),
-                 thetaOff = arena.add( theta );
+    const size_t copyOff = arena.add( copies ), stageOff = arena.add( stage ), evOff = arena.add( evict ), thetaOff = arena.add( theta );
+    std::vector<std::pair<int, size_t>> gramOffs;
+    for ( auto& kv : gram )
+        gramOffs.push_back( { kv.first, arena.add( kv.second ) } );
     std::vector<std::pair<std::pair<int, int>, size_t>> entryOffs;
     int nComb = 0;
     const size_t combOff = entry.upload( arena, entryOffs, nComb );
@@ -689,12 +692,9 @@ bool serverLaunchTrainRound( apsm_server* S )
         HIP_OR_THROW( hipGetLastError() );
     }
     launchRoundEntry( st, entry, base, entryOffs, combOff, nComb );
-    if ( !gram.empty() )
-    {
-        hipLaunchKernelGGL( apsm::batch_gram, dim3( gridFor( maxPairs, 256 ), (unsigned)gram.size() ), dim3( 256 ), 0, st,
-                            (const apsm::GramDesc*)( base + gramOff ) );
-        HIP_OR_THROW( hipGetLastError() );
-    }
+    for ( auto& go : gramOffs )
+        HIP_OR_THROW( apsm::launchBatchGramMfma( go.first, st, (const apsm::GramDesc*)( base + go.second ), (int)gram[ go.first ].size(),
+                                                 (int)( ( gramBlocksMax[ go.first ] + 3 ) / 4 ) ) );
     if ( !evict.empty() )
     {
         hipLaunchKernelGGL( apsm::batch_evict, dim3( gridFor( maxEv, 256 ), (unsigned)evict.size() ), dim3( 256 ), 0, st,

@@ -6,7 +6,7 @@
 //   apsm_entry_mfma    (whole GPU) for every entering complex sample t and every user:
 //          P_t = wL*thF.x_t + wG*sum_{j frozen, in the dictionary when t enters} c_j K(j,t),
 //          with thF = theta_0 + sum_{j < a0} c_j x_j.
-//   apsm_active_gram   kappa(t,j) = wL x_t.x_j + wG K(t,j) for every pair of active complex
+//   apsm_gram_mfma     kappa(t,j) = wL x_t.x_j + wG K(t,j) for every pair of active complex
 //          samples j <= t, as 3 floats per complex pair:
 //            A = kappa(a_t,a_j) = kappa(b_t,b_j), Bv = kappa(a_t,b_j), Cv = kappa(b_t,a_j).
 //   apsm_evict_corr    wG*sum_e c_e K(e,u) for the rows e leaving the dictionary at each
@@ -48,44 +48,6 @@ __device__ __forceinline__ void pairDots( const float* __restrict__ x, const flo
         sp = fmaf( xr, zi, sp );
         sp = fmaf( -xi, zr, sp );
     }
-}
-
-struct GramArgs
-{
-    int N, L;
-    int64_t c0; // local complex index of the first active sample
-    int64_t nA; // active complex samples
-    float wL, wG, g2;
-};
-
-// Gact[tri(i) + j] (3 floats) for active samples i >= j (relative to c0); one thread per pair idx.
-__device__ __forceinline__ void activeGramBody( const float* __restrict__ Z, const float* __restrict__ Zn, float* __restrict__ Gact,
-                                                const GramArgs& a, const int64_t idx )
-{
-    const int64_t total = triIdx( a.nA );
-    if ( idx >= total )
-        return;
-    int64_t i = (int64_t)( ( sqrt( 8.0 * (double)idx + 1.0 ) - 1.0 ) * 0.5 );
-    while ( triIdx( i + 1 ) <= idx )
-        i++;
-    while ( triIdx( i ) > idx )
-        i--;
-    const int64_t j = idx - triIdx( i );
-    const int64_t t = a.c0 + i, jj = a.c0 + j;
-    float p, sp;
-    pairDots( Z + t * a.L, Z + jj * a.L, a.N, p, sp );
-    const float nn = Zn[ t ] + Zn[ jj ];
-    const float E0 = gaussFromDist( nn - 2.f * p, a.g2 );
-    const float E1 = gaussFromDist( nn - 2.f * sp, a.g2 );
-    const float E2 = gaussFromDist( nn + 2.f * sp, a.g2 );
-    Gact[ idx * 3 + 0 ] = a.wL * p + a.wG * E0;
-    Gact[ idx * 3 + 1 ] = a.wL * sp + a.wG * E1;
-    Gact[ idx * 3 + 2 ] = -a.wL * sp + a.wG * E2;
-}
-
-__global__ void apsm_active_gram( const float* __restrict__ Z, const float* __restrict__ Zn, float* __restrict__ Gact, const GramArgs a )
-{
-    activeGramBody( Z, Zn, Gact, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x );
 }
 
 struct EvArgs
