@@ -333,7 +333,8 @@ bool blockedEligible( const apsm_params& p )
 constexpr int64_t kBlockRowsMax = 256; // real rows entering per block
 
 template <int TP, int NEWPER, int PMAX>
-void launchSeq2( apsm_handle* h, const float* Z, const float* Zn, const float* Y, float* C, const apsm::Seq2Args& s2, size_t lds )
+void launchSeq2( apsm_handle* h, const float* Z, const float* Zn, const float* Y, float* C, const float* Gact, const float* EvF,
+                 const apsm::Seq2Args& s2, size_t lds )
 {
     if ( s2.nParts > PMAX || PMAX * s2.Wc > apsm::kRedF )
         throw ApsmError { APSM_ERR_UNSUPPORTED, "window layout does not fit the blocked train path" };
@@ -341,23 +342,23 @@ void launchSeq2( apsm_handle* h, const float* Z, const float* Zn, const float* Y
         HIP_OR_THROW( hipFuncSetAttribute( (const void*)apsm::apsm_train_seq2<TP, NEWPER, PMAX>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds ) );
     hipLaunchKernelGGL( ( apsm::apsm_train_seq2<TP, NEWPER, PMAX> ), dim3( h->U ), dim3( apsm::kSeqThreads ), lds, h->stream, Z, Zn,
-                        Y, C, h->F.ptr, h->thOld.ptr, h->Pbuf.ptr, h->Gact.ptr, h->EvF.ptr, s2 );
+                        Y, C, h->F.ptr, h->thOld.ptr, h->Pbuf.ptr, Gact, EvF, s2 );
     HIP_OR_THROW( hipGetLastError() );
 }
 
 // Instantiations: (register triples per thread, upper bound of the t-slot part count).
 template <int NEWPER>
-void launchSeq2Tp( apsm_handle* h, int tp, const float* Z, const float* Zn, const float* Y, float* C, const apsm::Seq2Args& s2,
-                   size_t lds )
+void launchSeq2Tp( apsm_handle* h, int tp, const float* Z, const float* Zn, const float* Y, float* C, const float* Gact, const float* EvF,
+                   const apsm::Seq2Args& s2, size_t lds )
 {
     switch ( tp )
     {
-    case 2: launchSeq2<2, NEWPER, 24>( h, Z, Zn, Y, C, s2, lds ); break;
-    case 4: launchSeq2<4, NEWPER, 16>( h, Z, Zn, Y, C, s2, lds ); break;
-    case 7: launchSeq2<7, NEWPER, 12>( h, Z, Zn, Y, C, s2, lds ); break;
-    case 10: launchSeq2<10, NEWPER, 10>( h, Z, Zn, Y, C, s2, lds ); break;
-    case 16: launchSeq2<16, NEWPER, 8>( h, Z, Zn, Y, C, s2, lds ); break;
-    default: launchSeq2<27, NEWPER, 6>( h, Z, Zn, Y, C, s2, lds ); break;
+    case 2: launchSeq2<2, NEWPER, 24>( h, Z, Zn, Y, C, Gact, EvF, s2, lds ); break;
+    case 4: launchSeq2<4, NEWPER, 16>( h, Z, Zn, Y, C, Gact, EvF, s2, lds ); break;
+    case 7: launchSeq2<7, NEWPER, 12>( h, Z, Zn, Y, C, Gact, EvF, s2, lds ); break;
+    case 10: launchSeq2<10, NEWPER, 10>( h, Z, Zn, Y, C, Gact, EvF, s2, lds ); break;
+    case 16: launchSeq2<16, NEWPER, 8>( h, Z, Zn, Y, C, Gact, EvF, s2, lds ); break;
+    default: launchSeq2<27, NEWPER, 6>( h, Z, Zn, Y, C, Gact, EvF, s2, lds ); break;
     }
 }
 
@@ -415,6 +416,7 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
     if ( nParts > apsm::kMaxParts || nParts * tp < Wc )
         throw ApsmError { APSM_ERR_UNSUPPORTED, "window layout does not fit the blocked train path" };
 
+    const int nmax = nmaxOf( h->N );
     for ( int64_t k0 = 0; k0 < nStepsTotal; k0 += steps )
     {
         const int64_t n = std::min( steps, nStepsTotal - k0 );
@@ -448,13 +450,11 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
             // users in groups of 16 share one launch
             apsm::EntryArgs eu = ea;
             eu.U = std::min( 16, U - u0 );
-            const float* Cg = b.C.ptr + (size_t)u0 * ld;
             const float* thg = h->thOld.ptr + (size_t)u0 * h->L;
             float* Pg = h->Pbuf.ptr + (size_t)u0 * nNewC * 2;
             float* partG = h->Ppart.ptr + (size_t)u0 * nChunks * nNewC * 2;
-            if ( nChunks > 0 )
-                HIP_OR_THROW( apsm::launchEntryMfma( nmaxOf( h->N ), roundUpPow2Users( eu.U ), h->stream, b.Z.ptr, b.Zn.ptr, Cg, partG, eu,
-                                                     jBeg, nChunks ) );
+            HIP_OR_THROW( apsm::launchEntryMfma( nmax, roundUpPow2Users( eu.U ), h->stream, b.Z.ptr, b.Zn.ptr, b.C.ptr + (size_t)u0 * ld,
+                                                 partG, eu, jBeg, nChunks, 0, nChunks ) );
             HIP_OR_THROW( apsm::launchEntryCombine( h->stream, b.Z.ptr, thg, partG, Pg, eu, nChunks ) );
         }
 
@@ -466,7 +466,7 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
         ga.wL = wL;
         ga.wG = wG;
         ga.g2 = g2;
-        HIP_OR_THROW( apsm::launchGramMfma( nmaxOf( h->N ), h->stream, b.Z.ptr, b.Zn.ptr, h->Gact.ptr, ga ) );
+        HIP_OR_THROW( apsm::launchGramMfma( nmax, h->stream, b.Z.ptr, b.Zn.ptr, h->Gact.ptr, ga ) );
 
         const int64_t loLast = sLast > h->H ? sLast - h->H : 0, loPrev = sPrev > h->H ? sPrev - h->H : 0;
         const int hasEv = loLast > loPrev ? 1 : 0;
@@ -520,9 +520,9 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
         const float* Ya = b.Y.ptr + aR;
         float* Ca = b.C.ptr + aR;
         if ( newPer == 1 )
-            launchSeq2Tp<1>( h, tp, Za, Zna, Ya, Ca, s2, lds );
+            launchSeq2Tp<1>( h, tp, Za, Zna, Ya, Ca, h->Gact.ptr, h->EvF.ptr, s2, lds );
         else
-            launchSeq2Tp<2>( h, tp, Za, Zna, Ya, Ca, s2, lds );
+            launchSeq2Tp<2>( h, tp, Za, Zna, Ya, Ca, h->Gact.ptr, h->EvF.ptr, s2, lds );
         h->a0Next = a0Next;
     }
 }
@@ -962,6 +962,7 @@ void apsm_destroy( apsm_handle* h )
     h->Ppart.release();
     h->Gact.release();
     h->EvF.release();
+
     h->thOld.release();
     h->theta.release();
     h->rx.release();

@@ -33,7 +33,8 @@ struct EntryArgs
 // is cut into chunks of kEntryChunk entries (one workgroup each), then combined.
 constexpr int kEntryChunk = 128;
 
-// First frozen entry of the block's first entering sample and the number of chunks.
+// First frozen entry of the block's first entering sample and the number of chunks (chunk c
+// covers [j1 - (c+1) kEntryChunk, j1 - c kEntryChunk) within [jBeg, j1)).
 inline void entryChunks( const EntryArgs& a, int64_t& jBeg, int& nChunks )
 {
     const int64_t rt = a.rowBase + 2 * a.newC0;
@@ -102,11 +103,11 @@ hipError_t launchDetectMfma( int prec, int nmax, int umax, hipStream_t s, const 
                              const float* Zn, const float* C, int64_t ldC, const float* theta, int64_t G, int U, float wL,
                              float wG, float g2, float* est, int N );
 
-// Entry share of one block (nChunks > 0 chunk workgroups), then the combine of the chunks and the
+// Entry share of one block (chunks [chunk0, chunk0 + count) of nChunks), then the combine of the chunks and the
 // linear share into P [U][nNewC][2]; batched variants over n descriptors (gridX = most chunks,
 // resp. enough threads for the largest U * nNewC).
 hipError_t launchEntryMfma( int nmax, int umax, hipStream_t s, const float* Z, const float* Zn, const float* C, float* part,
-                            const EntryArgs& a, int64_t jBeg, int nChunks );
+                            const EntryArgs& a, int64_t jBeg, int nChunks, int chunk0, int count );
 hipError_t launchEntryCombine( hipStream_t s, const float* Z, const float* thF, const float* part, float* P, const EntryArgs& a,
                                int nChunks );
 hipError_t launchBatchEntryMfma( int nmax, int umax, hipStream_t s, const EntryMfmaDesc* d, int n, int gridX );

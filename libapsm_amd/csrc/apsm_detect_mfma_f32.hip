@@ -11,9 +11,11 @@ APSM_MFMA_ENTRY( mfmaEntryF32 )
 }
 
 hipError_t launchEntryMfma( int nmax, int umax, hipStream_t s, const float* Z, const float* Zn, const float* C, float* part,
-                            const EntryArgs& a, int64_t jBeg, int nChunks )
+                            const EntryArgs& a, int64_t jBeg, int nChunks, int chunk0, int count )
 {
-    return launchEntryAnyN( nmax, umax, false, s, Z, Zn, C, part, a, jBeg, nChunks, nullptr, 0, 0 );
+    if ( count <= 0 )
+        return hipSuccess;
+    return launchEntryAnyN( nmax, umax, false, s, Z, Zn, C, part, a, jBeg, nChunks, nullptr, chunk0, count );
 }
 
 hipError_t launchEntryCombine( hipStream_t s, const float* Z, const float* thF, const float* part, float* P, const EntryArgs& a,

@@ -5,7 +5,8 @@
 // and every user u: g_u(i) = sum over frozen dictionary entries j in [j0(i), j1) of
 // (c_a E0 + c_b E1, c_a E2 + c_b E0), where j1 is the block's frozen boundary and j0(i) the
 // dictionary's lower end when sample i enters (the FIFO may already have dropped entries).
-// The frozen range is cut into chunks of kEntryChunk entries; one workgroup takes one chunk
+// The frozen range is cut into chunks of kEntryChunk entries counted from its top (chunk 0 holds
+// the most recently frozen entries); one workgroup takes one chunk
 // for all (at most 128) entering samples, with the layout of the matrix-core detect (4 waves x
 // 32 samples, lane l holds sample l % 32 and entries 8(v/4) + 4(l/32) + v%4 of each 32-entry
 // block), masks the entries below j0(i), and writes its partial sums. A second kernel adds the
@@ -63,8 +64,9 @@ __device__ __forceinline__ void entryChunkMfma( const float* __restrict__ Z, con
     for ( int u = 0; u < UMAX; u++ )
         fa[ u ] = fb[ u ] = 0.f;
 
-    const int64_t c0 = jBeg + (int64_t)chunk * kEntryChunk;
-    const int64_t c1 = ( c0 + kEntryChunk ) < j1 ? c0 + kEntryChunk : j1;
+    // chunks from the top of the frozen range: chunk 0 holds the most recently frozen entries
+    const int64_t c1 = j1 - (int64_t)chunk * kEntryChunk;
+    const int64_t c0 = ( c1 - kEntryChunk ) > jBeg ? c1 - kEntryChunk : jBeg;
     for ( int64_t j0 = c0; j0 < c1; j0 += kTile )
     {
         const int nt = ( c1 - j0 ) < kTile ? (int)( c1 - j0 ) : kTile;
@@ -167,6 +169,7 @@ __device__ __forceinline__ void entryCombineBody( const float* __restrict__ Z, c
     }
     float ga = 0.f, gb = 0.f;
     const float2* pp = reinterpret_cast<const float2*>( part ) + (int64_t)u * nChunks * a.nNewC + i;
+#pragma unroll 8
     for ( int c = 0; c < nChunks; c++ )
     {
         const float2 v = pp[ (int64_t)c * a.nNewC ];
@@ -180,9 +183,9 @@ __device__ __forceinline__ void entryCombineBody( const float* __restrict__ Z, c
 template <int NMAX, int UMAX>
 __global__ void __launch_bounds__( kMfmaThreads ) apsm_entry_mfma( const float* __restrict__ Z, const float* __restrict__ Zn,
                                                                   const float* __restrict__ C, float* __restrict__ part, const EntryArgs a,
-                                                                  int64_t jBeg, int nChunks )
+                                                                  int64_t jBeg, int nChunks, int chunk0 )
 {
-    entryChunkMfma<NMAX, UMAX>( Z, Zn, C, part, a, jBeg, nChunks, blockIdx.x );
+    entryChunkMfma<NMAX, UMAX>( Z, Zn, C, part, a, jBeg, nChunks, chunk0 + blockIdx.x );
 }
 
 __global__ void apsm_entry_combine( const float* __restrict__ Z, const float* __restrict__ thF, const float* __restrict__ part,
@@ -206,6 +209,7 @@ __global__ void batch_entry_combine( const EntryMfmaDesc* __restrict__ d )
     entryCombineBody( e.Z, e.thF, e.part, e.P, e.a, e.nChunks, blockIdx.x * blockDim.x + threadIdx.x );
 }
 
+// Single handle: chunks [chunk0, chunk0 + gridX) of nChunks; batched: gridX = most chunks.
 template <int NMAX, int UMAX>
 hipError_t launchEntryNU( bool batch, hipStream_t s, const float* Z, const float* Zn, const float* C, float* part, const EntryArgs& a,
                           int64_t jBeg, int nChunks, const EntryMfmaDesc* d, int n, int gridX )
@@ -213,7 +217,7 @@ hipError_t launchEntryNU( bool batch, hipStream_t s, const float* Z, const float
     if ( batch )
         hipLaunchKernelGGL( ( batch_entry_mfma<NMAX, UMAX> ), dim3( gridX, n ), dim3( kMfmaThreads ), 0, s, d );
     else
-        hipLaunchKernelGGL( ( apsm_entry_mfma<NMAX, UMAX> ), dim3( nChunks ), dim3( kMfmaThreads ), 0, s, Z, Zn, C, part, a, jBeg, nChunks );
+        hipLaunchKernelGGL( ( apsm_entry_mfma<NMAX, UMAX> ), dim3( gridX ), dim3( kMfmaThreads ), 0, s, Z, Zn, C, part, a, jBeg, nChunks, n );
     return hipGetLastError();
 }
 

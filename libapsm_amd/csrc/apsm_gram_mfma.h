@@ -30,28 +30,26 @@ __device__ __forceinline__ void gramBlockMfma( const float* __restrict__ Z, cons
     const float* xi = Z + ( a.c0 + i ) * L;
     const float* zj = Z + ( a.c0 + jr ) * L;
 
-    mf32x16 accP = {}, accS = {};
+    // operands first (all loads in flight at once), then the MFMA chain
+    float xv[ K / 2 ], zr[ K / 2 ], zim[ K / 2 ];
 #pragma unroll
     for ( int t = 0; t < K / 2; t++ )
     {
         const int k = 2 * t + h;
         const int ant = k < NMAX ? k : k - NMAX;
-        float xv = 0.f, zp = 0.f, zs = 0.f;
-        if ( ant < N )
-        {
-            if ( vi )
-                xv = xi[ k < NMAX ? ant : N + ant ];
-            if ( vj )
-            {
-                const float zr = zj[ ant ], zim = zj[ N + ant ];
-                zp = k < NMAX ? zr : zim; // Zp row = [Re z | Im z]
-                zs = k < NMAX ? zim : -zr; // Zs row = [Im z | -Re z]
-            }
-        }
-        accP = __builtin_amdgcn_mfma_f32_32x32x2f32( zp, xv, accP, 0, 0, 0 );
-        accS = __builtin_amdgcn_mfma_f32_32x32x2f32( zs, xv, accS, 0, 0, 0 );
-        if ( ( t & 7 ) == 7 )
-            __builtin_amdgcn_sched_barrier( 0 );
+        xv[ t ] = ( ant < N && vi ) ? xi[ k < NMAX ? ant : N + ant ] : 0.f;
+        zr[ t ] = ( ant < N && vj ) ? zj[ ant ] : 0.f;
+        zim[ t ] = ( ant < N && vj ) ? zj[ N + ant ] : 0.f;
+    }
+    mf32x16 accP = {}, accS = {};
+#pragma unroll
+    for ( int t = 0; t < K / 2; t++ )
+    {
+        const bool re = 2 * t + h < NMAX;
+        const float zp = re ? zr[ t ] : zim[ t ]; // Zp row = [Re z | Im z]
+        const float zs = re ? zim[ t ] : -zr[ t ]; // Zs row = [Im z | -Re z]
+        accP = __builtin_amdgcn_mfma_f32_32x32x2f32( zp, xv[ t ], accP, 0, 0, 0 );
+        accS = __builtin_amdgcn_mfma_f32_32x32x2f32( zs, xv[ t ], accS, 0, 0, 0 );
     }
     if ( !vi )
         return;
