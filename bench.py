@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--serving-frames", type=int, default=600)
     ap.add_argument("--serving-links", type=int, default=2048)
+    ap.add_argument("--only", choices=["batched", "mixed", "serving"], default=None,
+                    help="run one further section alone (for profiling); prints {section: result}")
     ap.add_argument("--mixed-samples", type=int, default=None, help="override the mixed workload's detect samples")
     return ap.parse_args()
 
@@ -403,6 +405,16 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.only is not None:
+        if args.only == "batched":
+            res = bench_batched(args.batched_instances, max(1, args.steps))
+        elif args.only == "mixed":
+            res = bench_mixed(max(1, args.steps), args.mixed_samples)
+        else:
+            res = bench_serving(args.serving_frames, args.serving_links)
+        print(json.dumps({args.only: res}))
+        return
 
     from libapsm_amd import ApsmNomaDetectorFactory, LlsNomaDetector, datagen, lib
     from libapsm_amd._lib import DeviceArray, check
