@@ -63,6 +63,7 @@ void launchBatchDetectU( hipStream_t s, int umax, int threads, const apsm::DetDe
     case 2: launchBatchDetectT<NMAX, 2>( s, threads, d, items, nItems, grid ); break;
     case 4: launchBatchDetectT<NMAX, 4>( s, threads, d, items, nItems, grid ); break;
     case 8: launchBatchDetectT<NMAX, 8>( s, threads, d, items, nItems, grid ); break;
+    case 12: launchBatchDetectT<NMAX, 12>( s, threads, d, items, nItems, grid ); break;
     default: launchBatchDetectT<NMAX, 16>( s, threads, d, items, nItems, grid ); break;
     }
 }
@@ -269,7 +270,7 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
             const int nNewC = (int)( nb * newPer );
             const int nActC = (int)( ( sLast - a0 ) / 2 );
             TrainBlk k {};
-            k.umax = roundUpPow2Users( U );
+            k.umax = umaxOf( U );
             k.tp = tp;
             k.newPer = newPer;
             apsm::EntryArgs& ea = k.e.a;
@@ -520,7 +521,7 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
     std::map<std::array<int, 3>, std::vector<int>> dBuckets;
     for ( int i = 0; i < n; i++ )
         dBuckets[ { detectKernelOf( B->inst[ i ]->p.detect_precision, (uint32_t)det[ i ].N ), nmaxOf( det[ i ].N ),
-                    roundUpPow2Users( det[ i ].U ) } ]
+                    umaxOf( det[ i ].U ) } ]
             .push_back( i );
     const size_t detOff = arena.add( det );
     struct DetBucket

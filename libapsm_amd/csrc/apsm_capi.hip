@@ -132,7 +132,7 @@ constexpr float kLog2e = 1.4426950408889634f;
 
 // Template buckets of the antenna count and of a user group.
 int nmaxOf( int N ) { return N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64; }
-int roundUpPow2Users( int U ) { return U <= 1 ? 1 : U <= 2 ? 2 : U <= 4 ? 4 : U <= 8 ? 8 : 16; }
+int umaxOf( int U ) { return U <= 1 ? 1 : U <= 2 ? 2 : U <= 4 ? 4 : U <= 8 ? 8 : U <= 12 ? 12 : 16; }
 
 // APSM_PRECISION_FP32 runs the cross terms on the matrix cores from this many antennas on
 // (below, the vector-ALU kernel is at least as fast).
@@ -453,7 +453,7 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
             const float* thg = h->thOld.ptr + (size_t)u0 * h->L;
             float* Pg = h->Pbuf.ptr + (size_t)u0 * nNewC * 2;
             float* partG = h->Ppart.ptr + (size_t)u0 * nChunks * nNewC * 2;
-            HIP_OR_THROW( apsm::launchEntryMfma( nmax, roundUpPow2Users( eu.U ), h->stream, b.Z.ptr, b.Zn.ptr, b.C.ptr + (size_t)u0 * ld,
+            HIP_OR_THROW( apsm::launchEntryMfma( nmax, umaxOf( eu.U ), h->stream, b.Z.ptr, b.Zn.ptr, b.C.ptr + (size_t)u0 * ld,
                                                  partG, eu, jBeg, nChunks, 0, nChunks ) );
             HIP_OR_THROW( apsm::launchEntryCombine( h->stream, b.Z.ptr, thg, partG, Pg, eu, nChunks ) );
         }
@@ -744,6 +744,8 @@ void launchDetectU( apsm_handle* h, const float* d_rx, int64_t M, float* d_est, 
         launchDetect<NMAX, 4>( h, d_rx, M, d_est, Z, Zn, C, ldC, theta, G, U );
     else if ( U <= 8 )
         launchDetect<NMAX, 8>( h, d_rx, M, d_est, Z, Zn, C, ldC, theta, G, U );
+    else if ( U <= 12 )
+        launchDetect<NMAX, 12>( h, d_rx, M, d_est, Z, Zn, C, ldC, theta, G, U );
     else
         launchDetect<NMAX, 16>( h, d_rx, M, d_est, Z, Zn, C, ldC, theta, G, U );
 }
@@ -788,7 +790,7 @@ void detectDev( apsm_handle* h, const float* d_rx, int64_t M, float* d_est )
         if ( kernel >= 0 )
         {
             const float g2 = 0.5f / h->var * kLog2e;
-            HIP_OR_THROW( apsm::launchDetectMfma( kernel, nmaxOf( (int)h->N ), roundUpPow2Users( Ug ), h->stream, d_rx, M, Z, Zn, Cg, ld,
+            HIP_OR_THROW( apsm::launchDetectMfma( kernel, nmaxOf( (int)h->N ), umaxOf( Ug ), h->stream, d_rx, M, Z, Zn, Cg, ld,
                                                   th, G, Ug, h->p.linear_kernel_weight, h->p.gaussian_kernel_weight, g2, est,
                                                   (int)h->N ) );
         }

@@ -31,7 +31,7 @@ struct EntryArgs
 
 // Frozen-dictionary share of a block's entering pilots on the matrix cores: the frozen range
 // is cut into chunks of kEntryChunk entries (one workgroup each), then combined.
-constexpr int kEntryChunk = 128;
+constexpr int kEntryChunk = 64;
 
 // First frozen entry of the block's first entering sample and the number of chunks (chunk c
 // covers [j1 - (c+1) kEntryChunk, j1 - c kEntryChunk) within [jBeg, j1)).
@@ -98,7 +98,7 @@ struct WorkItem
     int tile;
 };
 
-// Single-handle detect of M samples; nmax in {4, 8, 16, 32, 64}, umax in {1, 2, 4, 8, 16}.
+// Single-handle detect of M samples; nmax in {4, 8, 16, 32, 64}, umax in {1, 2, 4, 8, 12, 16}.
 hipError_t launchDetectMfma( int prec, int nmax, int umax, hipStream_t s, const float* rx, int64_t M, const float* Z,
                              const float* Zn, const float* C, int64_t ldC, const float* theta, int64_t G, int U, float wL,
                              float wG, float g2, float* est, int N );

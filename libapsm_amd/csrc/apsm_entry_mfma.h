@@ -231,6 +231,7 @@ hipError_t launchEntryN( int umax, bool batch, hipStream_t s, const float* Z, co
     case 2: return launchEntryNU<NMAX, 2>( batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
     case 4: return launchEntryNU<NMAX, 4>( batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
     case 8: return launchEntryNU<NMAX, 8>( batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
+    case 12: return launchEntryNU<NMAX, 12>( batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
     case 16: return launchEntryNU<NMAX, 16>( batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
     default: return hipErrorInvalidValue;
     }

@@ -141,7 +141,7 @@ void planTrainCall( apsm_handle* h, const float* d_rx, const float* d_tx, int64_
         const int nNewC = (int)( n * newPer );
         const int nActC = (int)( ( sLast - a0 ) / 2 );
         TrainBlk k {};
-        k.umax = roundUpPow2Users( U );
+        k.umax = umaxOf( U );
         k.tp = tp;
         k.newPer = newPer;
         apsm::EntryArgs& ea = k.e.a;
@@ -522,7 +522,7 @@ bool serverLaunchDetect( apsm_server* S )
     {
         apsm_handle* h = S->links[ f->f.link ].h;
         det.push_back( detectDescOf( h, f->dRxD, f->f.n_data, f->dEst ) );
-        dBuckets[ { detectKernelOf( h->p.detect_precision, (uint32_t)h->N ), nmaxOf( h->N ), roundUpPow2Users( h->U ) } ].push_back(
+        dBuckets[ { detectKernelOf( h->p.detect_precision, (uint32_t)h->N ), nmaxOf( h->N ), umaxOf( h->U ) } ].push_back(
             (int)det.size() - 1 );
         f->detectBatch = id;
     }
