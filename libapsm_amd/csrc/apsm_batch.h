@@ -21,12 +21,6 @@ struct StageDesc
     int N, U;
 };
 
-struct EvDesc
-{
-    const float *Z, *Zn, *C;
-    float* EvF;
-    EvArgs a;
-};
 
 struct Seq2Desc
 {

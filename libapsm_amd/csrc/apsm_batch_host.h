@@ -146,48 +146,50 @@ struct DescArena
     }
 };
 
-// The entry share of a round: one chunked launch per (NMAX, UMAX) bucket, then one combine over
-// every descriptor of the round (all buckets' descriptors are contiguous at combOff).
-struct EntryLaunch
+// The block precompute of a round: one fused launch (entry chunks, Gram, eviction shares) per
+// (NMAX, UMAX) bucket, then one combine over every instance of the round.
+struct PreLaunch
 {
-    std::map<std::pair<int, int>, std::vector<apsm::EntryMfmaDesc>> buckets;
-    std::map<std::pair<int, int>, int> maxChunks;
+    std::map<std::pair<int, int>, std::vector<apsm::PreDesc>> buckets;
+    std::map<std::pair<int, int>, int> maxWg;
     int maxCombine = 0;
     void add( const TrainBlk& k )
     {
+        apsm::PreDesc d {};
+        d.e = k.e;
+        d.g = k.g;
+        d.v = k.v;
+        d.hasEv = k.hasEv ? 1 : 0;
+        apsm::preShape( d );
         const auto key = std::make_pair( k.nmax, k.umax );
-        buckets[ key ].push_back( k.e );
-        maxChunks[ key ] = std::max( maxChunks[ key ], k.e.nChunks );
+        buckets[ key ].push_back( d );
+        maxWg[ key ] = std::max( maxWg[ key ], d.entryWg + d.gramWg + d.evWg );
         maxCombine = std::max( maxCombine, k.e.a.U * k.e.a.nNewC );
     }
-    // descriptors into the arena: per bucket, then all of them again for the combine
+    // descriptors into the arena: per bucket, then every instance's entry descriptor for the combine
     size_t upload( DescArena& arena, std::vector<std::pair<std::pair<int, int>, size_t>>& offs, int& nComb ) const
     {
         std::vector<apsm::EntryMfmaDesc> all;
         for ( auto& kv : buckets )
         {
             offs.push_back( { kv.first, arena.add( kv.second ) } );
-            all.insert( all.end(), kv.second.begin(), kv.second.end() );
+            for ( auto& d : kv.second )
+                all.push_back( d.e );
         }
         nComb = (int)all.size();
         return arena.add( all );
     }
 };
 
-void launchRoundEntry( hipStream_t s, const EntryLaunch& el, const char* base, const std::vector<std::pair<std::pair<int, int>, size_t>>& offs,
-                       size_t combOff, int nComb )
+void launchRoundPre( hipStream_t s, const PreLaunch& pl, const char* base, const std::vector<std::pair<std::pair<int, int>, size_t>>& offs,
+                     size_t combOff, int nComb )
 {
     for ( auto& o : offs )
-    {
-        const int gx = el.maxChunks.at( o.first );
-        if ( gx > 0 )
-            HIP_OR_THROW( apsm::launchBatchEntryMfma( o.first.first, o.first.second, s, (const apsm::EntryMfmaDesc*)( base + o.second ),
-                                                      (int)el.buckets.at( o.first ).size(), gx ) );
-    }
+        HIP_OR_THROW( apsm::launchBatchBlockPre( o.first.first, o.first.second, s, (const apsm::PreDesc*)( base + o.second ),
+                                                 (int)pl.buckets.at( o.first ).size(), pl.maxWg.at( o.first ) ) );
     if ( nComb > 0 )
-        HIP_OR_THROW( apsm::launchBatchEntryCombine( s, (const apsm::EntryMfmaDesc*)( base + combOff ), nComb, ( el.maxCombine + 255 ) / 256 ) );
+        HIP_OR_THROW( apsm::launchBatchEntryCombine( s, (const apsm::EntryMfmaDesc*)( base + combOff ), nComb, ( pl.maxCombine + 255 ) / 256 ) );
 }
-
 
 void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT, const uint32_t* T, const float* const* rxD, const uint32_t* M,
                 float* const* est, bool useGraph )
@@ -418,11 +420,7 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
         maxBlocks = std::max( maxBlocks, v.size() );
     for ( size_t r = 0; r < maxBlocks; r++ )
     {
-        EntryLaunch entry;
-        std::map<int, std::vector<apsm::GramDesc>> gram; // by NMAX
-        std::map<int, int64_t> gramBlocksMax;
-        std::vector<apsm::EvDesc> evict;
-        int64_t maxEv = 0;
+        PreLaunch entry;
         std::map<std::pair<int, int>, std::vector<apsm::Seq2Desc>> seq;
         std::map<std::pair<int, int>, std::pair<int, size_t>> seqDim; // (max users, max lds)
         for ( int i = 0; i < n; i++ )
@@ -431,13 +429,6 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
                 continue;
             const TrainBlk& k = blocks[ i ][ r ];
             entry.add( k );
-            gram[ k.nmax ].push_back( k.g );
-            gramBlocksMax[ k.nmax ] = std::max( gramBlocksMax[ k.nmax ], apsm::gramBlocks( k.g.a.nA ) );
-            if ( k.hasEv )
-            {
-                evict.push_back( k.v );
-                maxEv = std::max<int64_t>( maxEv, (int64_t)k.v.a.nSteps * ( k.v.a.w2 / 2 ) );
-            }
             const auto key = std::make_pair( k.tp, k.newPer );
             seq[ key ].push_back( k.s );
             auto& dim = seqDim[ key ];
@@ -448,25 +439,7 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
             std::vector<std::pair<std::pair<int, int>, size_t>> offs;
             int nComb = 0;
             const size_t combOff = entry.upload( arena, offs, nComb );
-            pend.push_back( { 0, [ = ]( const char* base ) { launchRoundEntry( B->stream, entry, base, offs, combOff, nComb ); } } );
-        }
-        for ( auto& kv : gram )
-        {
-            const size_t off = arena.add( kv.second );
-            const int cnt = (int)kv.second.size(), nmax = kv.first, gx = (int)( ( gramBlocksMax[ kv.first ] + 3 ) / 4 );
-            pend.push_back( { 0, [ = ]( const char* base ) {
-                                 HIP_OR_THROW( apsm::launchBatchGramMfma( nmax, B->stream, (const apsm::GramDesc*)( base + off ), cnt, gx ) );
-                             } } );
-        }
-        if ( !evict.empty() )
-        {
-            const size_t off = arena.add( evict );
-            const int cnt = (int)evict.size(), gx = gridFor( maxEv, 256 );
-            pend.push_back( { 0, [ = ]( const char* base ) {
-                                 hipLaunchKernelGGL( apsm::batch_evict, dim3( gx, cnt ), dim3( 256 ), 0, B->stream,
-                                                     (const apsm::EvDesc*)( base + off ) );
-                                 HIP_OR_THROW( hipGetLastError() );
-                             } } );
+            pend.push_back( { 0, [ = ]( const char* base ) { launchRoundPre( B->stream, entry, base, offs, combOff, nComb ); } } );
         }
         // the round's sequencer buckets are independent: fork them onto side streams
         struct SeqLaunch

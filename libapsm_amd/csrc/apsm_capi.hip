@@ -445,18 +445,6 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
         int64_t jBeg = 0;
         int nChunks = 0;
         apsm::entryChunks( ea, jBeg, nChunks );
-        for ( int u0 = 0; u0 < U; u0 += 16 )
-        {
-            // users in groups of 16 share one launch
-            apsm::EntryArgs eu = ea;
-            eu.U = std::min( 16, U - u0 );
-            const float* thg = h->thOld.ptr + (size_t)u0 * h->L;
-            float* Pg = h->Pbuf.ptr + (size_t)u0 * nNewC * 2;
-            float* partG = h->Ppart.ptr + (size_t)u0 * nChunks * nNewC * 2;
-            HIP_OR_THROW( apsm::launchEntryMfma( nmax, umaxOf( eu.U ), h->stream, b.Z.ptr, b.Zn.ptr, b.C.ptr + (size_t)u0 * ld,
-                                                 partG, eu, jBeg, nChunks, 0, nChunks ) );
-            HIP_OR_THROW( apsm::launchEntryCombine( h->stream, b.Z.ptr, thg, partG, Pg, eu, nChunks ) );
-        }
 
         apsm::GramArgs ga {};
         ga.N = h->N;
@@ -466,29 +454,53 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
         ga.wL = wL;
         ga.wG = wG;
         ga.g2 = g2;
-        HIP_OR_THROW( apsm::launchGramMfma( nmax, h->stream, b.Z.ptr, b.Zn.ptr, h->Gact.ptr, ga ) );
-
         const int64_t loLast = sLast > h->H ? sLast - h->H : 0, loPrev = sPrev > h->H ? sPrev - h->H : 0;
         const int hasEv = loLast > loPrev ? 1 : 0;
-        if ( hasEv )
+        apsm::EvArgs ev {};
+        ev.N = h->N;
+        ev.L = h->L;
+        ev.U = U;
+        ev.q = (int)q;
+        ev.w2 = w2;
+        ev.nSteps = (int)n;
+        ev.rowBase = rowBase;
+        ev.sPrev = sPrev;
+        ev.sFirst = sFirst;
+        ev.H = h->H;
+        ev.ldYC = ld;
+        ev.wG = wG;
+        ev.g2 = g2;
+
+        // one launch for the entry share of the first 16 users, the Gram and the eviction
+        // shares; further user groups get entry launches of their own; then the combines
+        for ( int u0 = 0; u0 < U; u0 += 16 )
         {
-            apsm::EvArgs ev {};
-            ev.N = h->N;
-            ev.L = h->L;
-            ev.U = U;
-            ev.q = (int)q;
-            ev.w2 = w2;
-            ev.nSteps = (int)n;
-            ev.rowBase = rowBase;
-            ev.sPrev = sPrev;
-            ev.sFirst = sFirst;
-            ev.H = h->H;
-            ev.ldYC = ld;
-            ev.wG = wG;
-            ev.g2 = g2;
-            hipLaunchKernelGGL( apsm::apsm_evict_corr, dim3( gridFor( n * Wc, 256 ) ), dim3( 256 ), 0, h->stream, b.Z.ptr, b.Zn.ptr,
-                                b.C.ptr, h->EvF.ptr, ev );
-            HIP_OR_THROW( hipGetLastError() );
+            apsm::EntryArgs eu = ea;
+            eu.U = std::min( 16, U - u0 );
+            const float* Cg = b.C.ptr + (size_t)u0 * ld;
+            float* partG = h->Ppart.ptr + (size_t)u0 * nChunks * nNewC * 2;
+            if ( u0 == 0 )
+            {
+                apsm::PreDesc pd {};
+                pd.e = apsm::EntryMfmaDesc { b.Z.ptr, b.Zn.ptr, Cg, h->thOld.ptr, h->Pbuf.ptr, partG, eu, jBeg, nChunks };
+                pd.g = apsm::GramDesc { b.Z.ptr, b.Zn.ptr, h->Gact.ptr, ga };
+                pd.v = apsm::EvDesc { b.Z.ptr, b.Zn.ptr, b.C.ptr, h->EvF.ptr, ev };
+                pd.hasEv = hasEv;
+                apsm::preShape( pd );
+                HIP_OR_THROW( apsm::launchBlockPre( nmax, umaxOf( eu.U ), h->stream, pd ) );
+            }
+            else
+                HIP_OR_THROW( apsm::launchEntryMfma( nmax, umaxOf( eu.U ), h->stream, b.Z.ptr, b.Zn.ptr, Cg, partG, eu, jBeg, nChunks, 0,
+                                                     nChunks ) );
+        }
+        for ( int u0 = 0; u0 < U; u0 += 16 )
+        {
+            apsm::EntryArgs eu = ea;
+            eu.U = std::min( 16, U - u0 );
+            const float* thg = h->thOld.ptr + (size_t)u0 * h->L;
+            float* Pg = h->Pbuf.ptr + (size_t)u0 * nNewC * 2;
+            float* partG = h->Ppart.ptr + (size_t)u0 * nChunks * nNewC * 2;
+            HIP_OR_THROW( apsm::launchEntryCombine( h->stream, b.Z.ptr, thg, partG, Pg, eu, nChunks ) );
         }
 
         apsm::Seq2Args s2 {};

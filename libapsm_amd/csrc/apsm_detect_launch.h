@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "apsm_common.h"
+
 namespace apsm {
 
 // Input precision of the matrix-core cross terms (accumulation is FP32 for every one).
@@ -80,6 +82,25 @@ __host__ __device__ inline int64_t gramBlocks( int64_t nA )
     return nb * ( nb + 1 ) / 2;
 }
 
+// One train block's precompute as one launch: the entry-share chunks (entryWg workgroups), the
+// Gram blocks (gramWg workgroups of four 32 x 32 blocks) and the eviction shares (evWg
+// workgroups of 256 threads), each workgroup taking one role; the chunk combine follows.
+struct PreDesc
+{
+    EntryMfmaDesc e;
+    GramDesc g;
+    EvDesc v;
+    int hasEv;
+    int entryWg, gramWg, evWg;
+};
+
+inline void preShape( PreDesc& d )
+{
+    d.entryWg = d.e.nChunks;
+    d.gramWg = (int)( ( gramBlocks( d.g.a.nA ) + 3 ) / 4 );
+    d.evWg = d.hasEv ? (int)( ( (int64_t)d.v.a.nSteps * ( d.v.a.w2 / 2 ) + 255 ) / 256 ) : 0;
+}
+
 // One instance of a batched detect: its data samples, dictionary, coefficients and weights.
 struct DetDesc
 {
@@ -114,6 +135,8 @@ hipError_t launchBatchEntryMfma( int nmax, int umax, hipStream_t s, const EntryM
 hipError_t launchBatchEntryCombine( hipStream_t s, const EntryMfmaDesc* d, int n, int gridX );
 
 hipError_t launchGramMfma( int nmax, hipStream_t s, const float* Z, const float* Zn, float* Gact, const GramArgs& a );
+hipError_t launchBlockPre( int nmax, int umax, hipStream_t s, const PreDesc& d );
+hipError_t launchBatchBlockPre( int nmax, int umax, hipStream_t s, const PreDesc* d, int n, int gridX );
 hipError_t launchBatchGramMfma( int nmax, hipStream_t s, const GramDesc* d, int n, int gridX );
 
 // Persistent batched detect: `grid` workgroups walk nItems work items of tiles of kMfmaSamples.

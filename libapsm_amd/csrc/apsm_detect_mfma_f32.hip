@@ -44,6 +44,21 @@ hipError_t launchGramMfma( int nmax, hipStream_t s, const float* Z, const float*
     return launchGramAnyN( nmax, false, s, Z, Zn, Gact, a, nullptr, 0, 0 );
 }
 
+hipError_t launchBlockPre( int nmax, int umax, hipStream_t s, const PreDesc& d )
+{
+    const int grid = d.entryWg + d.gramWg + d.evWg;
+    if ( grid <= 0 )
+        return hipSuccess;
+    return launchPreAnyN( nmax, umax, false, s, &d, 1, grid );
+}
+
+hipError_t launchBatchBlockPre( int nmax, int umax, hipStream_t s, const PreDesc* d, int n, int gridX )
+{
+    if ( gridX <= 0 || n <= 0 )
+        return hipSuccess;
+    return launchPreAnyN( nmax, umax, true, s, d, n, gridX );
+}
+
 hipError_t launchBatchGramMfma( int nmax, hipStream_t s, const GramDesc* d, int n, int gridX )
 {
     return launchGramAnyN( nmax, true, s, nullptr, nullptr, nullptr, GramArgs {}, d, n, gridX );

@@ -22,41 +22,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "apsm_common.h"
+
 namespace apsm {
-
-constexpr int kWave = 64;
-constexpr int kSeqThreads = 1024; // train sequencer workgroup
-constexpr int kSeqWaves = kSeqThreads / kWave;
-constexpr int kNewMax = 4; // complex samples entering per chunk inside one step
-constexpr int kDetTile = 64; // dictionary entries per LDS tile in detect
-constexpr int kDetThreads = 256;
-
-// exp(-gamma*d) for d >= 0 via the hardware base-2 exponential; g2 = gamma*log2(e).
-__device__ __forceinline__ float gaussFromDist( float d, float g2 )
-{
-    return __builtin_amdgcn_exp2f( -g2 * fmaxf( d, 0.0f ) );
-}
-
-// Sum over the 64 lanes of a wave, returned in every lane. Within each row of 16 lanes the
-// DPP steps quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_ror:4 and row_ror:8 leave the row
-// sum in all 16 lanes; the four row sums are then read from lanes 0, 16, 32 and 48.
-__device__ __forceinline__ float waveSum( float v )
-{
-    int x = __float_as_int( v );
-    v += __int_as_float( __builtin_amdgcn_update_dpp( 0, x, 0xB1, 0xF, 0xF, false ) );
-    x = __float_as_int( v );
-    v += __int_as_float( __builtin_amdgcn_update_dpp( 0, x, 0x4E, 0xF, 0xF, false ) );
-    x = __float_as_int( v );
-    v += __int_as_float( __builtin_amdgcn_update_dpp( 0, x, 0x124, 0xF, 0xF, false ) );
-    x = __float_as_int( v );
-    v += __int_as_float( __builtin_amdgcn_update_dpp( 0, x, 0x128, 0xF, 0xF, false ) );
-    x = __float_as_int( v );
-    const float r0 = __int_as_float( __builtin_amdgcn_readlane( x, 0 ) );
-    const float r1 = __int_as_float( __builtin_amdgcn_readlane( x, 16 ) );
-    const float r2 = __int_as_float( __builtin_amdgcn_readlane( x, 32 ) );
-    const float r3 = __int_as_float( __builtin_amdgcn_readlane( x, 48 ) );
-    return ( r0 + r1 ) + ( r2 + r3 );
-}
 
 // ---------------------------------------------------------------------------------------
 // Staging: host layout rx [N][S][2] (+ optional permutation) -> stream rows Z [S][2N] and

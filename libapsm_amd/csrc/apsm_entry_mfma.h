@@ -14,6 +14,7 @@
 #pragma once
 
 #include "apsm_detect_mfma.h"
+#include "apsm_gram_mfma.h"
 
 namespace apsm {
 
@@ -210,6 +211,81 @@ __global__ void batch_entry_combine( const EntryMfmaDesc* __restrict__ d )
 }
 
 // Single handle: chunks [chunk0, chunk0 + gridX) of nChunks; batched: gridX = most chunks.
+// One block's precompute, workgroup b: entry chunk, Gram blocks or eviction shares.
+template <int NMAX, int UMAX>
+__device__ __forceinline__ void blockPreBody( const PreDesc& d, int b )
+{
+    if ( b < d.entryWg )
+    {
+        entryChunkMfma<NMAX, UMAX>( d.e.Z, d.e.Zn, d.e.C, d.e.part, d.e.a, d.e.jBeg, d.e.nChunks, b );
+        return;
+    }
+    b -= d.entryWg;
+    if ( b < d.gramWg )
+    {
+        const int64_t blk = (int64_t)b * ( kMfmaThreads / 64 ) + ( threadIdx.x >> 6 );
+        if ( blk < gramBlocks( d.g.a.nA ) )
+            gramBlockMfma<NMAX>( d.g.Z, d.g.Zn, d.g.Gact, d.g.a, blk );
+        return;
+    }
+    b -= d.gramWg;
+    if ( b < d.evWg )
+        evictCorrBody( d.v.Z, d.v.Zn, d.v.C, d.v.EvF, d.v.a, (int64_t)b * kMfmaThreads + threadIdx.x );
+}
+
+template <int NMAX, int UMAX>
+__global__ void __launch_bounds__( kMfmaThreads ) apsm_block_pre( const PreDesc d )
+{
+    blockPreBody<NMAX, UMAX>( d, blockIdx.x );
+}
+
+template <int NMAX, int UMAX>
+__global__ void __launch_bounds__( kMfmaThreads ) batch_block_pre( const PreDesc* __restrict__ d )
+{
+    const PreDesc& x = d[ blockIdx.y ];
+    if ( (int)blockIdx.x >= x.entryWg + x.gramWg + x.evWg )
+        return;
+    blockPreBody<NMAX, UMAX>( x, blockIdx.x );
+}
+
+template <int NMAX, int UMAX>
+hipError_t launchPreNU( bool batch, hipStream_t s, const PreDesc* d, int n, int gridX )
+{
+    if ( batch )
+        hipLaunchKernelGGL( ( batch_block_pre<NMAX, UMAX> ), dim3( gridX, n ), dim3( kMfmaThreads ), 0, s, d );
+    else
+        hipLaunchKernelGGL( ( apsm_block_pre<NMAX, UMAX> ), dim3( gridX ), dim3( kMfmaThreads ), 0, s, *d );
+    return hipGetLastError();
+}
+
+template <int NMAX>
+hipError_t launchPreN( int umax, bool batch, hipStream_t s, const PreDesc* d, int n, int gridX )
+{
+    switch ( umax )
+    {
+    case 1: return launchPreNU<NMAX, 1>( batch, s, d, n, gridX );
+    case 2: return launchPreNU<NMAX, 2>( batch, s, d, n, gridX );
+    case 4: return launchPreNU<NMAX, 4>( batch, s, d, n, gridX );
+    case 8: return launchPreNU<NMAX, 8>( batch, s, d, n, gridX );
+    case 12: return launchPreNU<NMAX, 12>( batch, s, d, n, gridX );
+    case 16: return launchPreNU<NMAX, 16>( batch, s, d, n, gridX );
+    default: return hipErrorInvalidValue;
+    }
+}
+
+inline hipError_t launchPreAnyN( int nmax, int umax, bool batch, hipStream_t s, const PreDesc* d, int n, int gridX )
+{
+    switch ( nmax )
+    {
+    case 4: return launchPreN<4>( umax, batch, s, d, n, gridX );
+    case 8: return launchPreN<8>( umax, batch, s, d, n, gridX );
+    case 16: return launchPreN<16>( umax, batch, s, d, n, gridX );
+    case 32: return launchPreN<32>( umax, batch, s, d, n, gridX );
+    case 64: return launchPreN<64>( umax, batch, s, d, n, gridX );
+    default: return hipErrorInvalidValue;
+    }
+}
+
 template <int NMAX, int UMAX>
 hipError_t launchEntryNU( bool batch, hipStream_t s, const float* Z, const float* Zn, const float* C, float* part, const EntryArgs& a,
                           int64_t jBeg, int nChunks, const EntryMfmaDesc* d, int n, int gridX )

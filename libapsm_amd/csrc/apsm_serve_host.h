@@ -623,11 +623,7 @@ bool serverLaunchTrainRound( apsm_server* S )
     S->pendingTrain.clear();
 
     DescArena arena;
-    EntryLaunch entry;
-    std::map<int, std::vector<apsm::GramDesc>> gram; // by NMAX
-    std::map<int, int64_t> gramBlocksMax;
-    int64_t maxEv = 0;
-    std::vector<apsm::EvDesc> evict;
+    PreLaunch entry;
     std::map<std::pair<int, int>, std::vector<apsm::Seq2Desc>> seq;
     std::map<std::pair<int, int>, std::pair<int, size_t>> seqDim;
     std::vector<apsm::ThetaDesc> theta;
@@ -637,13 +633,6 @@ bool serverLaunchTrainRound( apsm_server* S )
         ServeFrame* f = *it;
         const TrainBlk& k = f->plan.blocks[ f->nextBlock++ ];
         entry.add( k );
-        gram[ k.nmax ].push_back( k.g );
-        gramBlocksMax[ k.nmax ] = std::max( gramBlocksMax[ k.nmax ], apsm::gramBlocks( k.g.a.nA ) );
-        if ( k.hasEv )
-        {
-            evict.push_back( k.v );
-            maxEv = std::max<int64_t>( maxEv, (int64_t)k.v.a.nSteps * ( k.v.a.w2 / 2 ) );
-        }
         const auto key = std::make_pair( k.tp, k.newPer );
         seq[ key ].push_back( k.s );
         auto& dim = seqDim[ key ];
@@ -660,10 +649,7 @@ bool serverLaunchTrainRound( apsm_server* S )
         else
             ++it;
     }
-    const size_t copyOff = arena.add( copies ), stageOff = arena.add( stage ), evOff = arena.add( evict ), thetaOff = arena.add( theta );
-    std::vector<std::pair<int, size_t>> gramOffs;
-    for ( auto& kv : gram )
-        gramOffs.push_back( { kv.first, arena.add( kv.second ) } );
+    const size_t copyOff = arena.add( copies ), stageOff = arena.add( stage ), thetaOff = arena.add( theta );
     std::vector<std::pair<std::pair<int, int>, size_t>> entryOffs;
     int nComb = 0;
     const size_t combOff = entry.upload( arena, entryOffs, nComb );
@@ -691,16 +677,7 @@ bool serverLaunchTrainRound( apsm_server* S )
                             (const apsm::StageDesc*)( base + stageOff ) );
         HIP_OR_THROW( hipGetLastError() );
     }
-    launchRoundEntry( st, entry, base, entryOffs, combOff, nComb );
-    for ( auto& go : gramOffs )
-        HIP_OR_THROW( apsm::launchBatchGramMfma( go.first, st, (const apsm::GramDesc*)( base + go.second ), (int)gram[ go.first ].size(),
-                                                 (int)( ( gramBlocksMax[ go.first ] + 3 ) / 4 ) ) );
-    if ( !evict.empty() )
-    {
-        hipLaunchKernelGGL( apsm::batch_evict, dim3( gridFor( maxEv, 256 ), (unsigned)evict.size() ), dim3( 256 ), 0, st,
-                            (const apsm::EvDesc*)( base + evOff ) );
-        HIP_OR_THROW( hipGetLastError() );
-    }
+    launchRoundPre( st, entry, base, entryOffs, combOff, nComb );
     if ( !sl.empty() )
     {
         const int nside = std::min<int>( kBatchSideStreams, (int)sl.size() - 1 );
