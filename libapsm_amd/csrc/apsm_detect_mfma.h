@@ -48,17 +48,23 @@ __device__ __forceinline__ void detectTileMfma( const float* __restrict__ rx, in
                                                 float* __restrict__ est, int N, const int64_t tile )
 {
     constexpr int K = 2 * NMAX;
-    constexpr int kMfmaTile = NMAX >= 64 ? 32 : 64; // dictionary entries per LDS tile (<= 64 KB of LDS)
+    constexpr int kMfmaTile = NMAX >= 64 ? 32 : 64; // dictionary entries per LDS tile
     // 16-bit inputs: K padded to a multiple of 16 (one MFMA step), rows padded by 8 elements
     // (16 bytes) against LDS bank conflicts of the 16-byte reads
     constexpr int K16 = K < 16 ? 16 : K;
     constexpr int RowH = K16 + 8;
-    constexpr int kTileF = PREC == PREC_F32 ? 2 * K * kMfmaTile : 0; // f32: [2][K][tile] (transposed)
-    constexpr int kTileH = PREC == PREC_F32 ? 0 : 2 * kMfmaTile * RowH; // 16-bit: [2][tile][RowH]
-    __shared__ float sZf[ kTileF > 0 ? kTileF : 1 ];
-    __shared__ __attribute__( ( aligned( 16 ) ) ) uint16_t sZh[ kTileH > 0 ? kTileH : 1 ];
-    __shared__ float sZn[ kMfmaTile ];
-    __shared__ float2 sC[ UMAX * kMfmaTile ];
+    // Only Zp = [Re z | Im z] is staged; Zs = [Im z | -Re z] is the same row shifted by NMAX
+    // with the sign of its second half flipped. Two tile buffers: the next tile's loads are in
+    // flight (in registers) while the current tile is used, so a tile costs one barrier.
+    constexpr int kTileF = PREC == PREC_F32 ? K * kMfmaTile : 0; // f32: [K][tile] (transposed)
+    constexpr int kTileH = PREC == PREC_F32 ? 0 : kMfmaTile * RowH; // 16-bit: [tile][RowH]
+    constexpr int KL = PREC == PREC_F32 ? K : K16; // staged components per entry
+    constexpr int kLoads = ( KL * kMfmaTile + kMfmaThreads - 1 ) / kMfmaThreads; // per thread and tile
+    constexpr int kCLoads = ( UMAX * kMfmaTile + kMfmaThreads - 1 ) / kMfmaThreads;
+    __shared__ float sZf[ 2 ][ kTileF > 0 ? kTileF : 1 ];
+    __shared__ __attribute__( ( aligned( 16 ) ) ) uint16_t sZh[ 2 ][ kTileH > 0 ? kTileH : 1 ];
+    __shared__ float sZn[ 2 ][ kMfmaTile ];
+    __shared__ float2 sC[ 2 ][ UMAX * kMfmaTile ];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 31, h = lane >> 5;
@@ -98,56 +104,85 @@ __device__ __forceinline__ void detectTileMfma( const float* __restrict__ rx, in
     for ( int u = 0; u < UMAX; u++ )
         fa[ u ] = fb[ u ] = 0.f;
 
-    for ( int64_t j0 = 0; j0 < G; j0 += kMfmaTile )
-    {
-        const int nt = ( G - j0 ) < kMfmaTile ? (int)( G - j0 ) : kMfmaTile;
-        __syncthreads();
-        // tile: Zp and Zs rows of the entries (zeros past the dictionary end and past N)
-        if constexpr ( PREC == PREC_F32 )
+    // tile loads into registers: element r of this thread is component k, entry e with
+    // r * kMfmaThreads + tid = (f32) k * tile + e, (16-bit) e * K16 + k (zeros past the
+    // dictionary end, past N and, 16-bit, past K)
+    float zreg[ kLoads ];
+    float zn;
+    float2 creg[ kCLoads ];
+    auto loadTile = [ & ]( int64_t j0, int nt ) {
+#pragma unroll
+        for ( int r = 0; r < kLoads; r++ )
         {
-#pragma unroll 2
-            for ( int i = tid; i < K * kMfmaTile; i += kMfmaThreads )
-            {
-                const int k = i / kMfmaTile, e = i % kMfmaTile; // consecutive threads: consecutive entries
-                const int n = k < NMAX ? k : k - NMAX;
-                float zr = 0.f, zi = 0.f;
-                if ( e < nt && n < N )
-                {
-                    zr = Z[ ( j0 + e ) * L + n ];
-                    zi = Z[ ( j0 + e ) * L + N + n ];
-                }
-                sZf[ k * kMfmaTile + e ] = k < NMAX ? zr : zi; // Zp = [zr | zi]
-                sZf[ K * kMfmaTile + k * kMfmaTile + e ] = k < NMAX ? zi : -zr; // Zs = [zi | -zr]
-            }
+            const int i = r * kMfmaThreads + tid;
+            const int k = PREC == PREC_F32 ? i / kMfmaTile : i % K16;
+            const int e = PREC == PREC_F32 ? i % kMfmaTile : i / K16;
+            const int n = k < NMAX ? k : k - NMAX;
+            float v = 0.f;
+            if ( i < KL * kMfmaTile && e < nt && k < K && n < N )
+                v = Z[ ( j0 + e ) * L + ( k < NMAX ? n : N + n ) ];
+            zreg[ r ] = v;
         }
-        else
+        zn = tid < nt ? -g2 * Zn[ j0 + tid ] : 0.f;
+#pragma unroll
+        for ( int r = 0; r < kCLoads; r++ )
         {
-            typedef typename MfmaIn<PREC>::T T;
-#pragma unroll 2
-            for ( int i = tid; i < kMfmaTile * K16; i += kMfmaThreads )
-            {
-                const int e = i / K16, k = i % K16;
-                const int n = k < NMAX ? k : k - NMAX;
-                float zr = 0.f, zi = 0.f;
-                if ( e < nt && n < N && k < K )
-                {
-                    zr = Z[ ( j0 + e ) * L + n ];
-                    zi = Z[ ( j0 + e ) * L + N + n ];
-                }
-                const T vp = (T)( k < NMAX ? zr : zi ), vs = (T)( k < NMAX ? zi : -zr );
-                sZh[ e * RowH + k ] = __builtin_bit_cast( uint16_t, vp );
-                sZh[ kMfmaTile * RowH + e * RowH + k ] = __builtin_bit_cast( uint16_t, vs );
-            }
-        }
-        for ( int i = tid; i < kMfmaTile; i += kMfmaThreads )
-            sZn[ i ] = i < nt ? -g2 * Zn[ j0 + i ] : 0.f;
-        for ( int i = tid; i < UMAX * kMfmaTile; i += kMfmaThreads )
-        {
+            const int i = r * kMfmaThreads + tid;
             const int u = i / kMfmaTile, e = i % kMfmaTile;
-            sC[ i ] = ( u < U && e < nt ) ? make_float2( C[ u * ldC + 2 * ( j0 + e ) ], C[ u * ldC + 2 * ( j0 + e ) + 1 ] )
-                                          : make_float2( 0.f, 0.f );
+            creg[ r ] = ( i < UMAX * kMfmaTile && u < U && e < nt )
+                ? make_float2( C[ u * ldC + 2 * ( j0 + e ) ], C[ u * ldC + 2 * ( j0 + e ) + 1 ] )
+                : make_float2( 0.f, 0.f );
         }
-        __syncthreads();
+    };
+    auto storeTile = [ & ]( int buf ) {
+#pragma unroll
+        for ( int r = 0; r < kLoads; r++ )
+        {
+            const int i = r * kMfmaThreads + tid;
+            if ( i < KL * kMfmaTile )
+            {
+                if constexpr ( PREC == PREC_F32 )
+                    sZf[ buf ][ i ] = zreg[ r ];
+                else
+                {
+                    typedef typename MfmaIn<PREC>::T T;
+                    sZh[ buf ][ ( i / K16 ) * RowH + i % K16 ] = __builtin_bit_cast( uint16_t, (T)zreg[ r ] );
+                }
+            }
+        }
+        if ( tid < kMfmaTile )
+            sZn[ buf ][ tid ] = zn;
+#pragma unroll
+        for ( int r = 0; r < kCLoads; r++ )
+        {
+            const int i = r * kMfmaThreads + tid;
+            if ( i < UMAX * kMfmaTile )
+                sC[ buf ][ i ] = creg[ r ];
+        }
+    };
+
+    const int nTiles = (int)( ( G + kMfmaTile - 1 ) / kMfmaTile );
+    if ( nTiles > 0 )
+    {
+        loadTile( 0, G < kMfmaTile ? (int)G : kMfmaTile );
+        storeTile( 0 );
+    }
+    __syncthreads();
+#pragma unroll 1
+    for ( int tt = 0; tt < nTiles; tt++ )
+    {
+        const int buf = tt & 1;
+        const int64_t j0 = (int64_t)tt * kMfmaTile;
+        const int nt = ( G - j0 ) < kMfmaTile ? (int)( G - j0 ) : kMfmaTile;
+        if ( tt + 1 < nTiles )
+        {
+            const int64_t jn = j0 + kMfmaTile;
+            loadTile( jn, ( G - jn ) < kMfmaTile ? (int)( G - jn ) : kMfmaTile );
+        }
+        const float* zf = sZf[ buf ];
+        const uint16_t* zh = sZh[ buf ];
+        const float* zq = sZn[ buf ];
+        const float2* cc = sC[ buf ];
 
 #pragma unroll 1
         for ( int cb = 0; cb < nt; cb += 32 )
@@ -159,8 +194,9 @@ __device__ __forceinline__ void detectTileMfma( const float* __restrict__ rx, in
                 for ( int t = 0; t < K / 2; t++ )
                 {
                     const int k = 2 * t + h;
-                    const float ap = sZf[ k * kMfmaTile + cb + col ];
-                    const float as = sZf[ K * kMfmaTile + k * kMfmaTile + cb + col ];
+                    const float ap = zf[ k * kMfmaTile + cb + col ];
+                    // Zs component k: Im z_k (k < NMAX) or -Re z_(k-NMAX); k < NMAX for t < NMAX/2
+                    const float as = t < NMAX / 2 ? zf[ ( k + NMAX ) * kMfmaTile + cb + col ] : -zf[ ( k - NMAX ) * kMfmaTile + cb + col ];
                     accP = __builtin_amdgcn_mfma_f32_32x32x2f32( ap, xb[ t ], accP, 0, 0, 0 );
                     accS = __builtin_amdgcn_mfma_f32_32x32x2f32( as, xb[ t ], accS, 0, 0, 0 );
                     if ( ( t & 7 ) == 7 )
@@ -170,13 +206,34 @@ __device__ __forceinline__ void detectTileMfma( const float* __restrict__ rx, in
             else
             {
                 typedef typename MfmaIn<PREC>::V V;
+                typedef unsigned int u32x4 __attribute__( ( ext_vector_type( 4 ) ) );
 #pragma unroll
                 for ( int t = 0; t < K16 / 16; t++ )
                 {
                     const int k0 = 16 * t + 8 * h;
                     const V b = xv[ t ];
-                    const V ap = *reinterpret_cast<const V*>( &sZh[ ( cb + col ) * RowH + k0 ] );
-                    const V as = *reinterpret_cast<const V*>( &sZh[ kMfmaTile * RowH + ( cb + col ) * RowH + k0 ] );
+                    const uint16_t* row = zh + ( cb + col ) * RowH;
+                    const V ap = *reinterpret_cast<const V*>( row + k0 );
+                    // Zs: the row shifted by NMAX, second half negated (components past K meet
+                    // zeros of x, so whatever the shifted read holds there does not count)
+                    u32x4 bits;
+                    if constexpr ( NMAX % 8 == 0 )
+                    {
+                        // the lane's 8 components lie in one half: k0 < NMAX gives Im z, else -Re z
+                        const bool lowHalf = k0 < NMAX;
+                        bits = *reinterpret_cast<const u32x4*>( row + ( lowHalf ? k0 + NMAX : k0 - NMAX ) ); // 16-byte aligned
+                        bits ^= lowHalf ? 0u : 0x80008000u;
+                    }
+                    else
+                    {
+                        // NMAX = 4: components 0..3 are Im z (row + 4), 4..7 are -Re z (row + 0);
+                        // the lane with k0 = 8 is past K
+                        typedef unsigned int u32x2 __attribute__( ( ext_vector_type( 2 ) ) );
+                        const u32x2 lo = *reinterpret_cast<const u32x2*>( row + k0 + 4 );
+                        const u32x2 hi = *reinterpret_cast<const u32x2*>( row + k0 ) ^ 0x80008000u;
+                        bits = u32x4 { lo.x, lo.y, hi.x, hi.y };
+                    }
+                    const V as = __builtin_bit_cast( V, bits );
                     accP = MfmaIn<PREC>::mma( ap, b, accP );
                     accS = MfmaIn<PREC>::mma( as, b, accS );
                     if ( ( t & 1 ) == 1 )
@@ -187,7 +244,7 @@ __device__ __forceinline__ void detectTileMfma( const float* __restrict__ rx, in
             for ( int v = 0; v < 16; v++ )
             {
                 const int e = cb + 8 * ( v / 4 ) + 4 * h + ( v % 4 ); // entry within the tile
-                const float q = xq + sZn[ e ];
+                const float q = xq + zq[ e ];
                 const float p = accP[ v ], s = accS[ v ];
                 // exp2 of a clamped non-positive argument (a rounded distance below zero counts as 0)
                 const float E0 = __builtin_amdgcn_exp2f( fminf( fmaf( k2, p, q ), 0.f ) );
@@ -196,7 +253,7 @@ __device__ __forceinline__ void detectTileMfma( const float* __restrict__ rx, in
 #pragma unroll
                 for ( int u = 0; u < UMAX; u++ )
                 {
-                    const float2 c = sC[ u * kMfmaTile + e ];
+                    const float2 c = cc[ u * kMfmaTile + e ];
                     fa[ u ] = fmaf( c.x, E0, fmaf( c.y, E1, fa[ u ] ) );
                     fb[ u ] = fmaf( c.x, E2, fmaf( c.y, E0, fb[ u ] ) );
                 }
@@ -206,6 +263,9 @@ __device__ __forceinline__ void detectTileMfma( const float* __restrict__ rx, in
                     __builtin_amdgcn_sched_barrier( 0 );
             }
         }
+        if ( tt + 1 < nTiles )
+            storeTile( buf ^ 1 ); // the other buffer was released by the previous iteration's barrier
+        __syncthreads();
     }
 
     // the two lane halves saw disjoint entries of the same sample

@@ -57,8 +57,8 @@ def test_user_groups_equal_single_user_handles(require_gpu, mode, users):
 
 
 @pytest.mark.parametrize("mode", ["bf16", "fp16"])
-@pytest.mark.parametrize("N,mod,var", [(8, "QAM16", 0.05), (8, "QAM16", "sigma0.5"), (16, "QAM16", 0.05), (64, "QPSK", 0.05),
-                                       (64, "QPSK", "sigma0.5")])
+@pytest.mark.parametrize("N,mod,var", [(4, "QPSK", 0.05), (4, "QPSK", "sigma0.5"), (8, "QAM16", 0.05), (8, "QAM16", "sigma0.5"),
+                                       (16, "QAM16", 0.05), (32, "QAM16", "sigma0.5"), (64, "QPSK", 0.05), (64, "QPSK", "sigma0.5")])
 def test_half_precision_error_vs_oracle(require_gpu, oracle, mode, N, mod, var):
     """Six superimposed users at 12 dB so that symbol errors occur; the reference's default
     kernel variance (a wide kernel: every dictionary entry contributes) and sigma = 0.5 after
