@@ -330,7 +330,7 @@ bool blockedEligible( const apsm_params& p )
         && p.norm_constraint == 0.f;
 }
 
-constexpr int64_t kBlockRowsMax = 256; // real rows entering per block
+constexpr int64_t kBlockRowsMax = 1024; // real rows entering per block (the active-sample limit binds first)
 
 template <int TP, int NEWPER, int PMAX>
 void launchSeq2( apsm_handle* h, const float* Z, const float* Zn, const float* Y, float* C, const float* Gact, const float* EvF,
@@ -491,7 +491,7 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
             }
             else
                 HIP_OR_THROW( apsm::launchEntryMfma( nmax, umaxOf( eu.U ), h->stream, b.Z.ptr, b.Zn.ptr, Cg, partG, eu, jBeg, nChunks, 0,
-                                                     nChunks ) );
+                                                     nChunks * apsm::entryTiles( nNewC ) ) );
         }
         for ( int u0 = 0; u0 < U; u0 += 16 )
         {
@@ -1662,13 +1662,15 @@ int apsm_server_create( const apsm_server_config* cfg, apsm_server** out )
                 HIP_OR_THROW( hipEventCreateWithFlags( &S->join[ k ], hipEventDisableTiming ) );
             }
             HIP_OR_THROW( hipStreamCreateWithFlags( &S->dstream, hipStreamNonBlocking ) );
-            HIP_OR_THROW( hipStreamCreateWithFlags( &S->ustream, hipStreamNonBlocking ) );
+            for ( hipStream_t& u : S->ustream )
+                HIP_OR_THROW( hipStreamCreateWithFlags( &u, hipStreamNonBlocking ) );
             HIP_OR_THROW( hipEventCreateWithFlags( &S->fork, hipEventDisableTiming ) );
             for ( ServeRound& R : S->rounds )
                 HIP_OR_THROW( hipEventCreateWithFlags( &R.done, hipEventDisableTiming ) );
             for ( ServeDetect& D : S->detects )
             {
-                HIP_OR_THROW( hipEventCreateWithFlags( &D.uploaded, hipEventDisableTiming ) );
+                for ( hipEvent_t& e : D.uploaded )
+                    HIP_OR_THROW( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
                 HIP_OR_THROW( hipEventCreateWithFlags( &D.detected, hipEventDisableTiming ) );
             }
             int dev = 0;
@@ -1694,8 +1696,9 @@ void apsm_server_destroy( apsm_server* S )
         (void)hipStreamSynchronize( S->stream );
     if ( S->dstream )
         (void)hipStreamSynchronize( S->dstream );
-    if ( S->ustream )
-        (void)hipStreamSynchronize( S->ustream );
+    for ( hipStream_t u : S->ustream )
+        if ( u )
+            (void)hipStreamSynchronize( u );
     serverDropFrames( S );
     for ( ServeRound& R : S->rounds )
     {
@@ -1706,8 +1709,9 @@ void apsm_server_destroy( apsm_server* S )
     for ( ServeDetect& D : S->detects )
     {
         D.desc.release();
-        if ( D.uploaded )
-            (void)hipEventDestroy( D.uploaded );
+        for ( hipEvent_t e : D.uploaded )
+            if ( e )
+                (void)hipEventDestroy( e );
         if ( D.detected )
             (void)hipEventDestroy( D.detected );
     }
@@ -1726,8 +1730,9 @@ void apsm_server_destroy( apsm_server* S )
         (void)hipStreamDestroy( S->stream );
     if ( S->dstream )
         (void)hipStreamDestroy( S->dstream );
-    if ( S->ustream )
-        (void)hipStreamDestroy( S->ustream );
+    for ( hipStream_t u : S->ustream )
+        if ( u )
+            (void)hipStreamDestroy( u );
     delete S;
 }
 

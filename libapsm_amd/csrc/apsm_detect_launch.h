@@ -35,6 +35,9 @@ struct EntryArgs
 // is cut into chunks of kEntryChunk entries (one workgroup each), then combined.
 constexpr int kEntryChunk = 64;
 
+// Tiles of kMfmaSamples entering samples in the entry share (work items = tiles x chunks).
+__host__ __device__ inline int entryTiles( int nNewC ) { return ( nNewC + kMfmaSamples - 1 ) / kMfmaSamples; }
+
 // First frozen entry of the block's first entering sample and the number of chunks (chunk c
 // covers [j1 - (c+1) kEntryChunk, j1 - c kEntryChunk) within [jBeg, j1)).
 inline void entryChunks( const EntryArgs& a, int64_t& jBeg, int& nChunks )
@@ -96,7 +99,7 @@ struct PreDesc
 
 inline void preShape( PreDesc& d )
 {
-    d.entryWg = d.e.nChunks;
+    d.entryWg = d.e.nChunks * entryTiles( d.e.a.nNewC );
     d.gramWg = (int)( ( gramBlocks( d.g.a.nA ) + 3 ) / 4 );
     d.evWg = d.hasEv ? (int)( ( (int64_t)d.v.a.nSteps * ( d.v.a.w2 / 2 ) + 255 ) / 256 ) : 0;
 }

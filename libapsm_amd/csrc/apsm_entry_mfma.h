@@ -21,8 +21,10 @@ namespace apsm {
 template <int NMAX, int UMAX>
 __device__ __forceinline__ void entryChunkMfma( const float* __restrict__ Z, const float* __restrict__ Zn, const float* __restrict__ C,
                                                 float* __restrict__ part, const EntryArgs& a, const int64_t jBeg, const int nChunks,
-                                                const int chunk )
+                                                const int work )
 {
+    // work item = (tile of kMfmaSamples entering samples, chunk of the frozen range)
+    const int chunk = work % nChunks, mBase = ( work / nChunks ) * kMfmaSamples;
     constexpr int K = 2 * NMAX;
     constexpr int kTile = NMAX >= 64 ? 32 : 64;
     __shared__ float sZf[ 2 * K * kTile ];
@@ -32,7 +34,7 @@ __device__ __forceinline__ void entryChunkMfma( const float* __restrict__ Z, con
     const int N = a.N, L = a.L, U = a.U;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 31, h = lane >> 5;
-    const int m = wave * 32 + col; // entering sample of this lane
+    const int m = mBase + wave * 32 + col; // entering sample of this lane
     const bool valid = m < a.nNewC;
     const float g2 = a.g2;
 
@@ -199,7 +201,7 @@ template <int NMAX, int UMAX>
 __global__ void __launch_bounds__( kMfmaThreads ) batch_entry_mfma( const EntryMfmaDesc* __restrict__ d )
 {
     const EntryMfmaDesc& e = d[ blockIdx.y ];
-    if ( (int)blockIdx.x >= e.nChunks )
+    if ( (int)blockIdx.x >= e.nChunks * entryTiles( e.a.nNewC ) )
         return;
     entryChunkMfma<NMAX, UMAX>( e.Z, e.Zn, e.C, e.part, e.a, e.jBeg, e.nChunks, blockIdx.x );
 }
@@ -210,8 +212,9 @@ __global__ void batch_entry_combine( const EntryMfmaDesc* __restrict__ d )
     entryCombineBody( e.Z, e.thF, e.part, e.P, e.a, e.nChunks, blockIdx.x * blockDim.x + threadIdx.x );
 }
 
-// Single handle: chunks [chunk0, chunk0 + gridX) of nChunks; batched: gridX = most chunks.
-// One block's precompute, workgroup b: entry chunk, Gram blocks or eviction shares.
+// Single handle: work items [chunk0, chunk0 + gridX) of nChunks * entryTiles; batched: gridX =
+// most work items.
+// One block's precompute, workgroup b: entry work item, Gram blocks or eviction shares.
 template <int NMAX, int UMAX>
 __device__ __forceinline__ void blockPreBody( const PreDesc& d, int b )
 {

@@ -24,6 +24,7 @@
 
 constexpr int kServeRoundsInFlight = 2;
 constexpr int kServeDetectInFlight = 4;
+constexpr int kServeUploadStreams = 2;
 // A detect batch closes at this many frames or bytes, so that the upload of the next batch,
 // this batch's detect and the trains overlap instead of running as one serial burst.
 constexpr size_t kServeBatchFrames = 32;
@@ -353,7 +354,7 @@ struct DescSlot
 struct ServeDetect
 {
     int64_t id = -1;
-    hipEvent_t uploaded = nullptr, detected = nullptr;
+    hipEvent_t uploaded[ kServeUploadStreams ] = {}, detected = nullptr;
     std::vector<ServeFrame*> frames; // frames without pilots finish here
     DescSlot desc;
 };
@@ -375,7 +376,9 @@ struct apsm_server
     RangeAlloc arenaAlloc;
     hipStream_t stream = nullptr; // train rounds
     hipStream_t dstream = nullptr; // detect batches
-    hipStream_t ustream = nullptr; // frame uploads (overlap the previous batch's detect)
+    // frame uploads (overlap the previous batch's detect); frames alternate between the
+    // streams so that two copy engines share the host link
+    hipStream_t ustream[ kServeUploadStreams ] = {};
     hipStream_t side[ kBatchSideStreams ] = {};
     hipEvent_t fork = nullptr, join[ kBatchSideStreams ] = {};
     ServeRound rounds[ kServeRoundsInFlight ];
@@ -389,7 +392,7 @@ struct apsm_server
     uint64_t nextTicket = 0;
     std::chrono::steady_clock::time_point epoch;
     int cus = 256;
-    uint64_t roundsRun = 0, detectsRun = 0, maxBatch = 0;
+    uint64_t roundsRun = 0, detectsRun = 0, maxBatch = 0, uploads = 0;
 };
 
 namespace {
@@ -543,21 +546,33 @@ bool serverLaunchDetect( apsm_server* S )
                 items.push_back( apsm::WorkItem { i, (int)t } );
         detL.push_back( DetL { kernel, kv.first[ 1 ], kv.first[ 2 ], (int)items.size(), arena.add( items ) } );
     }
-    hipStream_t st = S->dstream, up = S->ustream;
-    const char* base = D.desc.upload( arena, up );
-    for ( ServeFrame* f : admitted )
+    // the large copies (data and pilot samples) alternate between the upload streams; the small
+    // ones (descriptors, transmitted pilots) go on the detect stream, where they do not
+    // interleave a copy kernel between the upload streams' engine copies
+    hipStream_t st = S->dstream;
+    const char* base = D.desc.upload( arena, st );
+    bool used[ kServeUploadStreams ] = {};
+    for ( size_t k = 0; k < admitted.size(); k++ )
     {
+        const ServeFrame* f = admitted[ k ];
         const apsm_handle* h = S->links[ f->f.link ].h;
         const size_t N = (size_t)h->N, U = (size_t)h->U;
+        const int ui = (int)( S->uploads++ % kServeUploadStreams );
+        hipStream_t up = S->ustream[ ui ];
+        used[ ui ] = true;
         HIP_OR_THROW( hipMemcpyAsync( f->dRxD, f->f.rx_data, N * f->f.n_data * 8, hipMemcpyHostToDevice, up ) );
         if ( f->f.n_pilots > 0 )
         {
             HIP_OR_THROW( hipMemcpyAsync( f->dRxP, f->f.rx_pilots, N * f->f.n_pilots * 8, hipMemcpyHostToDevice, up ) );
-            HIP_OR_THROW( hipMemcpyAsync( f->dTxP, f->f.tx_pilots, U * f->f.n_pilots * 8, hipMemcpyHostToDevice, up ) );
+            HIP_OR_THROW( hipMemcpyAsync( f->dTxP, f->f.tx_pilots, U * f->f.n_pilots * 8, hipMemcpyHostToDevice, st ) );
         }
     }
-    HIP_OR_THROW( hipEventRecord( D.uploaded, up ) );
-    HIP_OR_THROW( hipStreamWaitEvent( st, D.uploaded, 0 ) );
+    for ( int ui = 0; ui < kServeUploadStreams; ui++ )
+        if ( used[ ui ] )
+        {
+            HIP_OR_THROW( hipEventRecord( D.uploaded[ ui ], S->ustream[ ui ] ) );
+            HIP_OR_THROW( hipStreamWaitEvent( st, D.uploaded[ ui ], 0 ) );
+        }
     const apsm::DetDesc* dd = (const apsm::DetDesc*)( base + detOff );
     for ( const DetL& d : detL )
     {

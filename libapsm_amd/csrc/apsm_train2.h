@@ -74,7 +74,7 @@ __device__ __forceinline__ void waitVm()
     asm volatile( "s_waitcnt vmcnt(%0)" ::"n"( N ) : "memory" );
 }
 
-// s_waitcnt vmcnt(n) for a wave-uniform n <= 5
+// s_waitcnt vmcnt(min(n, 15)) for a wave-uniform n (a smaller count only waits longer)
 __device__ __forceinline__ void waitVmN( int n )
 {
     switch ( n )
@@ -84,7 +84,17 @@ __device__ __forceinline__ void waitVmN( int n )
     case 2: waitVm<2>(); break;
     case 3: waitVm<3>(); break;
     case 4: waitVm<4>(); break;
-    default: waitVm<5>(); break;
+    case 5: waitVm<5>(); break;
+    case 6: waitVm<6>(); break;
+    case 7: waitVm<7>(); break;
+    case 8: waitVm<8>(); break;
+    case 9: waitVm<9>(); break;
+    case 10: waitVm<10>(); break;
+    case 11: waitVm<11>(); break;
+    case 12: waitVm<12>(); break;
+    case 13: waitVm<13>(); break;
+    case 14: waitVm<14>(); break;
+    default: waitVm<15>(); break;
     }
 }
 
@@ -95,7 +105,7 @@ __device__ __forceinline__ void waitVmN( int n )
 // 32 bits; slots follow the absolute row (a0modW + rel) % w2, as the f array does.
 // Pointers are offset by the host: Z, Zn to the first active complex sample; Y, C to the
 // first active real row of user 0.
-// Step package (triple-buffered in LDS, copied two steps ahead by global->LDS DMA):
+// Step package (four buffers in LDS, copied three steps ahead by global->LDS DMA):
 //   [0, NEWPER*3*nAct)       Gram rows of the entering samples (row i at i*3*nAct)
 //   [+0, +2*NEWPER)          their frozen shares P
 //   [+0, +w2)                eviction shares of the window rows
@@ -107,6 +117,7 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
 {
     constexpr int kDpw = ( NEWPER * 3 * kActMax + 2 * NEWPER + kSeqThreads + kSeqThreads - 1 ) / kSeqThreads; // DMAs per wave per step
     constexpr int kPkCap = kDpw * kSeqThreads; // floats per package buffer
+    constexpr int kPkBufs = 4; // package buffers: step k's package is issued at step k - 3
     constexpr int TPA = ( TP + 3 ) & ~3; // column segment per part, padded for float4 reads
     constexpr int kColStride = kMaxParts * TPA; // floats per column array (A, B or C)
     const int tid = threadIdx.x, lane = tid & ( kWave - 1 ), wave = tid / kWave;
@@ -121,8 +132,8 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     const int pkF = pkRows + 2 * NEWPER + w2;
 
     extern __shared__ float sm[];
-    float* sPk = sm; // [3][kPkCap] (first, so the DMA destinations are 256-byte aligned)
-    float* sF = sPk + 3 * kPkCap; // [w2]
+    float* sPk = sm; // [kPkBufs][kPkCap] (first, so the DMA destinations are 256-byte aligned)
+    float* sF = sPk + kPkBufs * kPkCap; // [w2]
     float* sB = sF + w2; // [2 * kSeqThreads]
     float* sC = sB + 2 * kSeqThreads; // [2 * kActMax]
     float* sY = sC + 2 * kActMax; // [2 * kActMax]
@@ -137,13 +148,13 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     const float* EvU = EvF + (int64_t)u * a.nSteps * w2;
     const float* gEnd = Gact + ( nAct * ( nAct + 1 ) / 2 ) * 3 - 1;
 
-    // issue the package of step kk into buffer kk % 3; wave w copies 64-float chunks
+    // issue the package of step kk into buffer kk % kPkBufs; wave w copies 64-float chunks
     // w, w + 16, ... of the package (myChunks of them)
     const int nChunks = ( pkF + kWave - 1 ) / kWave;
     const int myChunks = wave < nChunks ? ( nChunks - wave + kSeqWaves - 1 ) / kSeqWaves : 0;
     auto pkIssue = [ & ]( int kk ) {
         const int c0 = a.newRel0 + kk * NEWPER;
-        float* buf = sPk + ( kk % 3 ) * kPkCap;
+        float* buf = sPk + ( kk % kPkBufs ) * kPkCap;
 #pragma unroll 1
         for ( int d = 0; d < myChunks; d++ )
         {
@@ -166,9 +177,8 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
         }
     };
 
-    pkIssue( 0 );
-    if ( a.nSteps > 1 )
-        pkIssue( 1 );
+    for ( int k = 0; k < kPkBufs - 1 && k < a.nSteps; k++ )
+        pkIssue( k );
 
     for ( int i = tid; i < w2; i += kSeqThreads )
         sF[ i ] = Fg[ (int64_t)u * w2 + i ];
@@ -213,53 +223,22 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
                 __builtin_amdgcn_sched_barrier( 0 );
         }
     }
-    waitVm<0>();
-    __syncthreads();
 
-    for ( int kk = 0; kk < a.nSteps; kk++ )
-    {
-        const float* pk = sPk + ( kk % 3 ) * kPkCap;
-        const int sRel = a.sFirstRel + kk * a.q; // s - a0
-        const int spRel = sRel - a.q;
-        const int W = a.ramp ? ( sRel < w2 ? sRel : w2 ) : w2;
-        const int c0 = a.newRel0 + kk * NEWPER; // first entering complex sample
-        const int sBase = ( a.a0modW + sRel ) % w2; // slot of row s (== slot of row s - w2)
-        // the package two steps ahead goes into the buffer read one step ago
-        if ( kk + 2 < a.nSteps )
-            pkIssue( kk + 2 );
-
-        // 1. previous step's Gram product lands in f; rows leaving the dictionary subtract their
-        //    share from the rows that were already in the window. Thread tid owns slot tid.
-        if ( tid < w2 )
+    // Work of step ks that does not depend on the projection of step ks - 1 beyond c: waves
+    // 8..15 take the active rows' share of each entering f (the c of rows that entered up to
+    // step ks - 1 are final for it), waves 5..7 stage the Gram column of each entering sample.
+    // It runs beside the Gram product of step ks - 1.
+    auto stage = [ & ]( int ks ) {
+        const float* pk = sPk + ( ks % kPkBufs ) * kPkCap;
+        const int c0 = a.newRel0 + ks * NEWPER; // first entering complex sample
+        if ( wave >= kDotBase / kWave )
         {
-            const int cv = tid >> 1, pt = tid & 1;
-            // parts beyond nParts hold zeros: the sum runs over PMAX without guards so the
-            // loads issue back to back
-            float acc = 0.f;
-#pragma unroll
-            for ( int t = 0; t < PMAX; t++ )
-                acc += sRedF[ 2 * ( t * Wc + cv ) + pt ];
-            if ( a.hasEv )
-            {
-                int pos = tid - sBase; // position of this slot's row among the last w2 rows
-                pos += pos < 0 ? w2 : 0;
-                const int r = sRel - w2 + pos;
-                const int wpos = r - ( sRel - W );
-                if ( wpos >= 0 && r < spRel )
-                    acc -= pk[ pkRows + 2 * NEWPER + wpos ];
-            }
-            sF[ tid ] += acc;
-        }
-        // 2. entering samples: share of the active rows with non-zero c, and Gram entries
-        {
-            // threads [512, 512 + c0) (waves 8..15) take one active sample each, so this runs
-            // beside the f reduction (waves 0..4) and the column staging (waves 5..7)
             float ea[ NEWPER ], eb[ NEWPER ];
 #pragma unroll
             for ( int i = 0; i < NEWPER; i++ )
                 ea[ i ] = eb[ i ] = 0.f;
             const int jd = tid - kDotBase;
-            if ( jd >= 0 && jd < c0 )
+            if ( jd < c0 )
             {
                 const float ca = sC[ 2 * jd ], cb = sC[ 2 * jd + 1 ];
 #pragma unroll
@@ -281,10 +260,106 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
                 }
             }
         }
-        // Gram entries of each entering sample k against the window as it stands after the
-        // step (rows entering later in the same step included): the row in complex slot x is
-        // kRef - ((pRef - x) mod Wc) with kRef the last entering sample. kappa(k, r) comes from
-        // the package row of max(k, r) at the position of min(k, r).
+        else if ( tid >= kColBase )
+        {
+            // column: sCol[i][.][part*TPA + j] = (kappa(a_tau,a_k), kappa(a_tau,b_k), kappa(b_tau,a_k))
+            // for tau = part*TP + j (segments padded to TPA floats for aligned vector reads),
+            // against the window as it stands after the step (rows entering later in the same
+            // step included): the row in complex slot x is kRef - ((pRef - x) mod Wc) with kRef
+            // the last entering sample; kappa(k, r) comes from the package row of max(k, r) at
+            // the position of min(k, r)
+            const int kRef = c0 + NEWPER - 1;
+            const int pRef = ( cMod + kRef ) % Wc;
+            const int ct = tid - kColBase; // t-slot staged by this thread
+            if ( ct < a.nParts * TP )
+            {
+                const int cpart = ct / TP, cj = ct - cpart * TP;
+                const int cdst = cpart * TPA + cj;
+                int dt = pRef - ct;
+                dt += dt < 0 ? Wc : 0;
+                const int rt = kRef - dt;
+                const bool ok = ct < Wc && rt >= 0;
+#pragma unroll
+                for ( int i = 0; i < NEWPER; i++ )
+                {
+                    const int k = c0 + i;
+                    float* col = sCol + i * 3 * kColStride;
+                    const bool later = ok && rt > k; // entering after k in this step
+                    const float* g = later ? pk + ( rt - c0 ) * rowFloats + 3 * k : pk + i * rowFloats + 3 * ( ok ? rt : 0 );
+                    const float m = ok ? 1.f : 0.f;
+                    col[ cdst ] = m * g[ 0 ];
+                    col[ kColStride + cdst ] = m * ( later ? g[ 1 ] : g[ 2 ] );
+                    col[ 2 * kColStride + cdst ] = m * ( later ? g[ 2 ] : g[ 1 ] );
+                }
+            }
+        }
+    };
+
+    waitVm<0>();
+    __syncthreads();
+    stage( 0 );
+    ldsBarrier();
+
+    for ( int kk = 0; kk < a.nSteps; kk++ )
+    {
+        const float* pk = sPk + ( kk % kPkBufs ) * kPkCap;
+        const int sRel = a.sFirstRel + kk * a.q; // s - a0
+        const int spRel = sRel - a.q;
+        const int W = a.ramp ? ( sRel < w2 ? sRel : w2 ) : w2;
+        const int c0 = a.newRel0 + kk * NEWPER; // first entering complex sample
+        const int sBase = ( a.a0modW + sRel ) % w2; // slot of row s (== slot of row s - w2)
+        // the package kPkBufs - 1 steps ahead goes into the buffer read in the previous step
+        if ( kk + kPkBufs - 1 < a.nSteps )
+            pkIssue( kk + kPkBufs - 1 );
+
+        // 1. projection over the window. Thread tid < w2 takes row r = s - w2 + tid (the last
+        //    w2 rows cover every slot once): its f gets the previous step's Gram product and,
+        //    for a row already in the window, loses the share of rows leaving the dictionary;
+        //    an entering row's f is its frozen share plus the active rows' share.
+        if ( tid < w2 )
+        {
+            const int r = sRel - w2 + tid;
+            int sl = sBase + tid;
+            sl -= sl >= w2 ? w2 : 0;
+            const int cv = sl >> 1, pt = sl & 1;
+            // parts beyond nParts hold zeros: the sum runs over PMAX without guards so the
+            // loads issue back to back
+            float acc = 0.f;
+#pragma unroll
+            for ( int t = 0; t < PMAX; t++ )
+                acc += sRedF[ 2 * ( t * Wc + cv ) + pt ];
+            const int wpos = r - ( sRel - W );
+            if ( a.hasEv && wpos >= 0 && r < spRel )
+                acc -= pk[ pkRows + 2 * NEWPER + wpos ];
+            float f = sF[ sl ] + acc;
+            float beta = 0.f;
+            if ( wpos >= 0 )
+            {
+                if ( r >= spRel )
+                {
+                    const int e2 = r - spRel; // 2*entering index + part
+                    float e = pk[ pkRows + e2 ];
+#pragma unroll
+                    for ( int w = kDotBase / kWave; w < kSeqWaves; w++ )
+                        e += sRedE[ w * 2 * kNewMax + e2 ];
+                    f = e;
+                }
+                const float res = sY[ r ] - f;
+                float b = 0.f;
+                if ( res > a.eB )
+                    b = res - a.eB;
+                else if ( res < -a.eB )
+                    b = res + a.eB;
+                b = b * ( 1.f / (float)W ) * sInv[ r >> 1 ];
+                beta = b;
+                sC[ r ] += b;
+            }
+            sF[ sl ] = f;
+            sB[ sl ] = beta; // sl = 2*complex slot + part
+        }
+        // 2. the entering samples' row and column of the register Gram: the threads whose
+        //    t-slots contain p take kappa(k, v) (rows first), then the thread holding u-slot p
+        //    takes the column staged beside the previous step's product
         {
             const int kRef = c0 + NEWPER - 1;
             const int pRef = ( cMod + kRef ) % Wc;
@@ -293,26 +368,6 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
             {
                 const int k = c0 + i;
                 const int p = ( cMod + k ) % Wc;
-                float* col = sCol + i * 3 * kColStride;
-                // column: sCol[i][.][part*TPA + j] = (kappa(a_tau,a_k), kappa(a_tau,b_k), kappa(b_tau,a_k))
-                // for tau = part*TP + j (segments padded to TPA floats for aligned vector reads)
-                const int ct = tid - kColBase; // t-slot staged by this thread
-                if ( ct >= 0 && ct < a.nParts * TP )
-                {
-                    const int cpart = ct / TP, cj = ct - cpart * TP;
-                    const int cdst = cpart * TPA + cj;
-                    int dt = pRef - ct;
-                    dt += dt < 0 ? Wc : 0;
-                    const int rt = kRef - dt;
-                    const bool ok = ct < Wc && rt >= 0;
-                    const bool later = ok && rt > k; // entering after k in this step
-                    const float* g = later ? pk + ( rt - c0 ) * rowFloats + 3 * k : pk + i * rowFloats + 3 * ( ok ? rt : 0 );
-                    const float m = ok ? 1.f : 0.f;
-                    col[ cdst ] = m * g[ 0 ];
-                    col[ kColStride + cdst ] = m * ( later ? g[ 1 ] : g[ 2 ] );
-                    col[ 2 * kColStride + cdst ] = m * ( later ? g[ 2 ] : g[ 1 ] );
-                }
-                // row p: the threads whose t-slots contain p take kappa(k, v)
                 const int jj = p - tBase;
                 if ( gThread && jj >= 0 && jj < TP )
                 {
@@ -341,72 +396,40 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
                     }
                 }
             }
-        }
-        ldsBarrier(); // A
-        // column p of each entering sample: the thread holding u-slot p takes its t-slots
 #pragma unroll
-        for ( int i = 0; i < NEWPER; i++ )
-        {
-            const int p = ( cMod + c0 + i ) % Wc;
-            if ( gThread && v == p )
+            for ( int i = 0; i < NEWPER; i++ )
             {
-                const float4* col = reinterpret_cast<const float4*>( sCol + i * 3 * kColStride + part * TPA );
-#pragma unroll
-                for ( int j4 = 0; j4 < TPA / 4; j4++ )
+                const int p = ( cMod + c0 + i ) % Wc;
+                if ( gThread && v == p )
                 {
-                    const float4 xa = col[ j4 ], xb = col[ kColStride / 4 + j4 ], xc = col[ 2 * kColStride / 4 + j4 ];
-                    const float va[ 4 ] = { xa.x, xa.y, xa.z, xa.w }, vb[ 4 ] = { xb.x, xb.y, xb.z, xb.w },
-                                vc[ 4 ] = { xc.x, xc.y, xc.z, xc.w };
+                    const float4* col = reinterpret_cast<const float4*>( sCol + i * 3 * kColStride + part * TPA );
 #pragma unroll
-                    for ( int q4 = 0; q4 < 4; q4++ )
+                    for ( int j4 = 0; j4 < TPA / 4; j4++ )
                     {
-                        if ( 4 * j4 + q4 < TP )
+                        const float4 xa = col[ j4 ], xb = col[ kColStride / 4 + j4 ], xc = col[ 2 * kColStride / 4 + j4 ];
+                        const float va[ 4 ] = { xa.x, xa.y, xa.z, xa.w }, vb[ 4 ] = { xb.x, xb.y, xb.z, xb.w },
+                                    vc[ 4 ] = { xc.x, xc.y, xc.z, xc.w };
+#pragma unroll
+                        for ( int q4 = 0; q4 < 4; q4++ )
                         {
-                            gA[ 4 * j4 + q4 ] = va[ q4 ];
-                            gB[ 4 * j4 + q4 ] = vb[ q4 ];
-                            gC[ 4 * j4 + q4 ] = vc[ q4 ];
+                            if ( 4 * j4 + q4 < TP )
+                            {
+                                gA[ 4 * j4 + q4 ] = va[ q4 ];
+                                gB[ 4 * j4 + q4 ] = vb[ q4 ];
+                                gC[ 4 * j4 + q4 ] = vc[ q4 ];
+                            }
                         }
                     }
                 }
             }
         }
-        // 3. projection weights over the window (entering rows complete their f first)
-        if ( tid < w2 )
-        {
-            const float invW = 1.f / (float)W;
-            const int r = sRel - w2 + tid; // the last w2 rows cover every slot once
-            int sl = sBase + tid;
-            sl -= sl >= w2 ? w2 : 0;
-            float beta = 0.f;
-            if ( r >= sRel - W )
-            {
-                float f;
-                if ( r >= spRel )
-                {
-                    const int e2 = r - spRel; // 2*entering index + part
-                    float acc = pk[ pkRows + e2 ];
-#pragma unroll
-                    for ( int w = 0; w < kSeqWaves; w++ )
-                        acc += sRedE[ w * 2 * kNewMax + e2 ];
-                    f = acc;
-                    sF[ sl ] = f;
-                }
-                else
-                    f = sF[ sl ];
-                const float res = sY[ r ] - f;
-                float b = 0.f;
-                if ( res > a.eB )
-                    b = res - a.eB;
-                else if ( res < -a.eB )
-                    b = res + a.eB;
-                b = b * invW * sInv[ r >> 1 ];
-                beta = b;
-                sC[ r ] += b;
-            }
-            sB[ sl ] = beta; // sl = 2*complex slot + part
-        }
+        // the next step's package (issued kPkBufs - 2 steps ago) must have landed: the staging
+        // of the next step below reads it
+        waitVmN( myChunks * ( ( kk + 2 < a.nSteps ? 1 : 0 ) + ( kk + 3 < a.nSteps ? 1 : 0 ) ) );
         ldsBarrier(); // B
-        // 4. partial f_v += sum_t kappa(t, v) beta_t from the register Gram (summed over the
+        if ( kk + 1 < a.nSteps )
+            stage( kk + 1 );
+        // 3. partial f_v += sum_t kappa(t, v) beta_t from the register Gram (summed over the
         //    t-slot parts at the start of the next step)
         if ( gThread )
         {
@@ -424,8 +447,6 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
             sRedF[ 2 * tid ] = fa;
             sRedF[ 2 * tid + 1 ] = fb;
         }
-        // the next step's package (issued two steps ago) must have landed
-        waitVmN( kk + 2 < a.nSteps ? myChunks : 0 );
         ldsBarrier(); // C
     }
     if ( tid < w2 )
@@ -485,7 +506,7 @@ inline size_t seq2LdsBytes( int newPer, int w2, int tp )
 {
     const int dpw = ( newPer * 3 * kActMax + 2 * newPer + kSeqThreads + kSeqThreads - 1 ) / kSeqThreads;
     const int tpa = ( tp + 3 ) & ~3;
-    const size_t floats = 3 * (size_t)dpw * kSeqThreads + w2 + 2 * kSeqThreads + 4 * kActMax + kActMax + kSeqWaves * 2 * kNewMax
+    const size_t floats = 4 * (size_t)dpw * kSeqThreads + w2 + 2 * kSeqThreads + 4 * kActMax + kActMax + kSeqWaves * 2 * kNewMax
         + 2 * kRedF + (size_t)newPer * 3 * kMaxParts * tpa;
     return floats * sizeof( float );
 }

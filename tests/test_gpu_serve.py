@@ -8,6 +8,9 @@
   test_gpu_parity (soft outputs within 2e-3, decisions equal away from boundaries).
 * Admission: links by HBM budget (APSM_ERR_BUSY), frames by arena room (they wait, results
   unchanged); the submit/poll interface gives the replay's results.
+* Frames in pinned host memory (moved by the server's upload kernel, estimates written by the
+  detect into host memory) give bitwise the results of the same frames in pageable memory
+  (engine copies), with 16-byte aligned and unaligned buffers.
 """
 import numpy as np
 import pytest
@@ -167,3 +170,36 @@ def test_frame_without_pilots_and_errors(require_gpu):
     res = srv.replay(frames)
     assert all(r["status"] == 0 for r in res)
     assert all(np.all(f.est == 0) for f in frames)  # no train yet: zero weights, empty dictionary
+
+
+def pinned_copy(a, offset_elems=0):
+    """A pinned copy of a complex64 array; offset_elems > 0 places it 8*offset bytes into the
+    allocation (so the buffer is not 16-byte aligned)."""
+    from libapsm_amd import PinnedArray
+    a = np.ascontiguousarray(a, dtype=np.complex64)
+    pa = PinnedArray((a.size + offset_elems,), np.complex64)
+    view = pa.array[offset_elems:].reshape(a.shape)
+    view[...] = a
+    return pa, view
+
+
+def test_pinned_frames_give_the_pageable_results(require_gpu):
+    from libapsm_amd import Frame
+    fr, links = trace(4, 24, rate_hz=20000.0, seed=16)
+    a = frames_of(fr)
+    make_server(links, cfg_of).replay(a)
+    keep = []
+    b = []
+    for k, f in enumerate(fr):
+        off = k % 2  # odd frames: every buffer 8 bytes off 16-byte alignment
+        rxd, rxd_v = pinned_copy(f["rx_data"], off)
+        rxp, rxp_v = pinned_copy(f["rx_pilots"], off)
+        txp, txp_v = pinned_copy(f["tx_pilots"], off)
+        est, est_v = pinned_copy(np.zeros((1, f["samples"]), np.complex64), off)
+        keep += [rxd, rxp, txp, est]
+        b.append(Frame(f["link"], rxd_v, rxp_v, txp_v, arrival_s=f["arrival_s"], est=est_v))
+    res = make_server(links, cfg_of).replay(b)
+    assert all(r["status"] == 0 for r in res)
+    for k, (x, y) in enumerate(zip(a, b)):
+        np.testing.assert_array_equal(y.est[0], x.est[0], err_msg=f"frame {k}: pinned vs pageable")
+    assert any(np.any(y.est != 0) for y in b)
