@@ -151,7 +151,8 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     // issue the package of step kk into buffer kk % kPkBufs; wave w copies 64-float chunks
     // w, w + 16, ... of the package (myChunks of them)
     const int nChunks = ( pkF + kWave - 1 ) / kWave;
-    const int myChunks = wave < nChunks ? ( nChunks - wave + kSeqWaves - 1 ) / kSeqWaves : 0;
+    // wave-uniform (readfirstlane): the copy loop and the vmcnt waits below branch on scalars
+    const int myChunks = __builtin_amdgcn_readfirstlane( wave < nChunks ? ( nChunks - wave + kSeqWaves - 1 ) / kSeqWaves : 0 );
     auto pkIssue = [ & ]( int kk ) {
         const int c0 = a.newRel0 + kk * NEWPER;
         float* buf = sPk + ( kk % kPkBufs ) * kPkCap;
@@ -381,18 +382,20 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
                     const float A = m * g[ 0 ];
                     const float Bv = m * ( later ? g[ 2 ] : g[ 1 ] );
                     const float Cv = m * ( later ? g[ 1 ] : g[ 2 ] );
-                    // per-lane select with static register indices; the empty asm keeps the
-                    // compiler from turning this into a dynamically indexed vector insert
+                    // the active lanes all belong to the part holding slot p, so jj is uniform
+                    // among them: a scalar test per register index (the empty asm keeps each
+                    // write behind its branch instead of a select over every register)
+                    const int jjs = __builtin_amdgcn_readfirstlane( jj );
 #pragma unroll
                     for ( int j = 0; j < TP; j++ )
                     {
-                        float xa = j == jj ? A : gA[ j ];
-                        float xb = j == jj ? Bv : gB[ j ];
-                        float xc = j == jj ? Cv : gC[ j ];
-                        asm volatile( "" : "+v"( xa ), "+v"( xb ), "+v"( xc ) );
-                        gA[ j ] = xa;
-                        gB[ j ] = xb;
-                        gC[ j ] = xc;
+                        if ( j == jjs )
+                        {
+                            gA[ j ] = A;
+                            gB[ j ] = Bv;
+                            gC[ j ] = Cv;
+                            asm volatile( "" : "+v"( gA[ j ] ), "+v"( gB[ j ] ), "+v"( gC[ j ] ) );
+                        }
                     }
                 }
             }
