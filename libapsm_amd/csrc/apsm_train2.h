@@ -105,7 +105,8 @@ __device__ __forceinline__ void waitVmN( int n )
 // 32 bits; slots follow the absolute row (a0modW + rel) % w2, as the f array does.
 // Pointers are offset by the host: Z, Zn to the first active complex sample; Y, C to the
 // first active real row of user 0.
-// Step package (four buffers in LDS, copied three steps ahead by global->LDS DMA):
+// Step package (four buffers in LDS, copied by global->LDS DMA during the product three steps
+// ahead):
 //   [0, NEWPER*3*nAct)       Gram rows of the entering samples (row i at i*3*nAct)
 //   [+0, +2*NEWPER)          their frozen shares P
 //   [+0, +w2)                eviction shares of the window rows
@@ -148,18 +149,20 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     const float* EvU = EvF + (int64_t)u * a.nSteps * w2;
     const float* gEnd = Gact + ( nAct * ( nAct + 1 ) / 2 ) * 3 - 1;
 
-    // issue the package of step kk into buffer kk % kPkBufs; wave w copies 64-float chunks
-    // w, w + 16, ... of the package (myChunks of them)
+    // issue the package of step kk into buffer kk % kPkBufs. The waves below kColBase (they
+    // reduce f and take the product, but do not stage) copy it, after their product, while the
+    // staging waves finish: wave w copies 64-float chunks w, w + kIssueWaves, ... (myChunks)
+    constexpr int kIssueWaves = kColBase / kWave;
     const int nChunks = ( pkF + kWave - 1 ) / kWave;
     // wave-uniform (readfirstlane): the copy loop and the vmcnt waits below branch on scalars
-    const int myChunks = __builtin_amdgcn_readfirstlane( wave < nChunks ? ( nChunks - wave + kSeqWaves - 1 ) / kSeqWaves : 0 );
+    const int myChunks = __builtin_amdgcn_readfirstlane( wave < kIssueWaves && wave < nChunks ? ( nChunks - wave + kIssueWaves - 1 ) / kIssueWaves : 0 );
     auto pkIssue = [ & ]( int kk ) {
         const int c0 = a.newRel0 + kk * NEWPER;
         float* buf = sPk + ( kk % kPkBufs ) * kPkCap;
 #pragma unroll 1
         for ( int d = 0; d < myChunks; d++ )
         {
-            const int chunk = wave + d * kSeqWaves;
+            const int chunk = wave + d * kIssueWaves;
             const int e = chunk * kWave + lane;
             const float* src = Gact;
             if ( e < pkRows )
@@ -309,9 +312,6 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
         const int W = a.ramp ? ( sRel < w2 ? sRel : w2 ) : w2;
         const int c0 = a.newRel0 + kk * NEWPER; // first entering complex sample
         const int sBase = ( a.a0modW + sRel ) % w2; // slot of row s (== slot of row s - w2)
-        // the package kPkBufs - 1 steps ahead goes into the buffer read in the previous step
-        if ( kk + kPkBufs - 1 < a.nSteps )
-            pkIssue( kk + kPkBufs - 1 );
 
         // 1. projection over the window. Thread tid < w2 takes row r = s - w2 + tid (the last
         //    w2 rows cover every slot once): its f gets the previous step's Gram product and,
@@ -426,9 +426,9 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
                 }
             }
         }
-        // the next step's package (issued kPkBufs - 2 steps ago) must have landed: the staging
-        // of the next step below reads it
-        waitVmN( myChunks * ( ( kk + 2 < a.nSteps ? 1 : 0 ) + ( kk + 3 < a.nSteps ? 1 : 0 ) ) );
+        // the next step's package (issued two steps ago) must have landed: the staging of the
+        // next step below reads it; the one issued in the previous step may still be in flight
+        waitVmN( kk + 2 < a.nSteps ? myChunks : 0 );
         ldsBarrier(); // B
         if ( kk + 1 < a.nSteps )
             stage( kk + 1 );
@@ -450,6 +450,9 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
             sRedF[ 2 * tid ] = fa;
             sRedF[ 2 * tid + 1 ] = fb;
         }
+        // the package kPkBufs - 1 steps ahead goes into the buffer read up to the previous step
+        if ( kk + kPkBufs - 1 < a.nSteps )
+            pkIssue( kk + kPkBufs - 1 );
         ldsBarrier(); // C
     }
     if ( tid < w2 )
