@@ -1655,22 +1655,25 @@ int apsm_server_create( const apsm_server_config* cfg, apsm_server** out )
         {
             S->cfg = *cfg;
             S->epoch = std::chrono::steady_clock::now();
+            // creation order: detect, upload, train, side (one hardware queue each)
+            HIP_OR_THROW( hipStreamCreateWithFlags( &S->dstream, hipStreamNonBlocking ) );
+            HIP_OR_THROW( hipStreamCreateWithFlags( &S->ustream, hipStreamNonBlocking ) );
             HIP_OR_THROW( hipStreamCreateWithFlags( &S->stream, hipStreamNonBlocking ) );
-            for ( int k = 0; k < kBatchSideStreams; k++ )
+            for ( int k = 0; k < kServeSideStreams; k++ )
             {
                 HIP_OR_THROW( hipStreamCreateWithFlags( &S->side[ k ], hipStreamNonBlocking ) );
                 HIP_OR_THROW( hipEventCreateWithFlags( &S->join[ k ], hipEventDisableTiming ) );
             }
-            HIP_OR_THROW( hipStreamCreateWithFlags( &S->dstream, hipStreamNonBlocking ) );
-            for ( hipStream_t& u : S->ustream )
-                HIP_OR_THROW( hipStreamCreateWithFlags( &u, hipStreamNonBlocking ) );
             HIP_OR_THROW( hipEventCreateWithFlags( &S->fork, hipEventDisableTiming ) );
             for ( ServeRound& R : S->rounds )
+            {
                 HIP_OR_THROW( hipEventCreateWithFlags( &R.done, hipEventDisableTiming ) );
+                R.desc.reserve( kServeDescBytes );
+            }
             for ( ServeDetect& D : S->detects )
             {
-                for ( hipEvent_t& e : D.uploaded )
-                    HIP_OR_THROW( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
+                HIP_OR_THROW( hipEventCreateWithFlags( &D.uploaded, hipEventDisableTiming ) );
+                D.desc.reserve( kServeDescBytes );
                 HIP_OR_THROW( hipEventCreateWithFlags( &D.detected, hipEventDisableTiming ) );
             }
             int dev = 0;
@@ -1696,9 +1699,8 @@ void apsm_server_destroy( apsm_server* S )
         (void)hipStreamSynchronize( S->stream );
     if ( S->dstream )
         (void)hipStreamSynchronize( S->dstream );
-    for ( hipStream_t u : S->ustream )
-        if ( u )
-            (void)hipStreamSynchronize( u );
+    if ( S->ustream )
+        (void)hipStreamSynchronize( S->ustream );
     serverDropFrames( S );
     for ( ServeRound& R : S->rounds )
     {
@@ -1709,15 +1711,14 @@ void apsm_server_destroy( apsm_server* S )
     for ( ServeDetect& D : S->detects )
     {
         D.desc.release();
-        for ( hipEvent_t e : D.uploaded )
-            if ( e )
-                (void)hipEventDestroy( e );
+        if ( D.uploaded )
+            (void)hipEventDestroy( D.uploaded );
         if ( D.detected )
             (void)hipEventDestroy( D.detected );
     }
     for ( ServeLink& L : S->links )
         apsm_destroy( L.h );
-    for ( int k = 0; k < kBatchSideStreams; k++ )
+    for ( int k = 0; k < kServeSideStreams; k++ )
     {
         if ( S->side[ k ] )
             (void)hipStreamDestroy( S->side[ k ] );
@@ -1730,9 +1731,8 @@ void apsm_server_destroy( apsm_server* S )
         (void)hipStreamDestroy( S->stream );
     if ( S->dstream )
         (void)hipStreamDestroy( S->dstream );
-    for ( hipStream_t u : S->ustream )
-        if ( u )
-            (void)hipStreamDestroy( u );
+    if ( S->ustream )
+        (void)hipStreamDestroy( S->ustream );
     delete S;
 }
 

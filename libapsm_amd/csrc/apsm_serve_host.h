@@ -24,7 +24,12 @@
 
 constexpr int kServeRoundsInFlight = 2;
 constexpr int kServeDetectInFlight = 4;
-constexpr int kServeUploadStreams = 2;
+// One stream each for detect batches, uploads, train rounds and one side stream for a round's
+// further sequencer buckets: four streams, created in that order, so that each gets a hardware
+// queue of its own (a process has four; streams beyond share them round-robin, and a detect
+// batch behind a train round's sequencer on a shared queue would wait for it).
+constexpr int kServeSideStreams = 1;
+constexpr size_t kServeDescBytes = size_t( 1 ) << 20; // descriptor staging per batch / round
 // A detect batch closes at this many frames or bytes, so that the upload of the next batch,
 // this batch's detect and the trains overlap instead of running as one serial burst.
 constexpr size_t kServeBatchFrames = 32;
@@ -324,6 +329,14 @@ struct DescSlot
     DevBuf<char> dev;
     char* host = nullptr;
     size_t hostCap = 0;
+    // Allocates the staging at server creation: growing it later frees memory, and the
+    // runtime's frees wait for every stream of the device (a stall of the whole pipeline).
+    void reserve( size_t bytes )
+    {
+        HIP_OR_THROW( hipHostMalloc( (void**)&host, bytes, hipHostMallocDefault ) );
+        hostCap = bytes;
+        dev.ensure( bytes );
+    }
     const char* upload( const DescArena& a, hipStream_t st )
     {
         const size_t bytes = std::max<size_t>( a.bytes.size(), 256 );
@@ -354,7 +367,7 @@ struct DescSlot
 struct ServeDetect
 {
     int64_t id = -1;
-    hipEvent_t uploaded[ kServeUploadStreams ] = {}, detected = nullptr;
+    hipEvent_t uploaded = nullptr, detected = nullptr;
     std::vector<ServeFrame*> frames; // frames without pilots finish here
     DescSlot desc;
 };
@@ -376,11 +389,9 @@ struct apsm_server
     RangeAlloc arenaAlloc;
     hipStream_t stream = nullptr; // train rounds
     hipStream_t dstream = nullptr; // detect batches
-    // frame uploads (overlap the previous batch's detect); frames alternate between the
-    // streams so that two copy engines share the host link
-    hipStream_t ustream[ kServeUploadStreams ] = {};
-    hipStream_t side[ kBatchSideStreams ] = {};
-    hipEvent_t fork = nullptr, join[ kBatchSideStreams ] = {};
+    hipStream_t ustream = nullptr; // frame uploads (overlap the previous batch's detect)
+    hipStream_t side[ kServeSideStreams ] = {};
+    hipEvent_t fork = nullptr, join[ kServeSideStreams ] = {};
     ServeRound rounds[ kServeRoundsInFlight ];
     ServeDetect detects[ kServeDetectInFlight ];
     std::deque<int64_t> roundsInFlight, detectsInFlight; // ids, oldest first
@@ -392,7 +403,7 @@ struct apsm_server
     uint64_t nextTicket = 0;
     std::chrono::steady_clock::time_point epoch;
     int cus = 256;
-    uint64_t roundsRun = 0, detectsRun = 0, maxBatch = 0, uploads = 0;
+    uint64_t roundsRun = 0, detectsRun = 0, maxBatch = 0;
 };
 
 namespace {
@@ -546,20 +557,15 @@ bool serverLaunchDetect( apsm_server* S )
                 items.push_back( apsm::WorkItem { i, (int)t } );
         detL.push_back( DetL { kernel, kv.first[ 1 ], kv.first[ 2 ], (int)items.size(), arena.add( items ) } );
     }
-    // the large copies (data and pilot samples) alternate between the upload streams; the small
-    // ones (descriptors, transmitted pilots) go on the detect stream, where they do not
-    // interleave a copy kernel between the upload streams' engine copies
-    hipStream_t st = S->dstream;
+    // the large copies (data and pilot samples) go on the upload stream; the small ones
+    // (descriptors, transmitted pilots) on the detect stream, where the runtime's copy kernel
+    // for them does not interleave with the upload stream's engine copies
+    hipStream_t st = S->dstream, up = S->ustream;
     const char* base = D.desc.upload( arena, st );
-    bool used[ kServeUploadStreams ] = {};
-    for ( size_t k = 0; k < admitted.size(); k++ )
+    for ( const ServeFrame* f : admitted )
     {
-        const ServeFrame* f = admitted[ k ];
         const apsm_handle* h = S->links[ f->f.link ].h;
         const size_t N = (size_t)h->N, U = (size_t)h->U;
-        const int ui = (int)( S->uploads++ % kServeUploadStreams );
-        hipStream_t up = S->ustream[ ui ];
-        used[ ui ] = true;
         HIP_OR_THROW( hipMemcpyAsync( f->dRxD, f->f.rx_data, N * f->f.n_data * 8, hipMemcpyHostToDevice, up ) );
         if ( f->f.n_pilots > 0 )
         {
@@ -567,12 +573,8 @@ bool serverLaunchDetect( apsm_server* S )
             HIP_OR_THROW( hipMemcpyAsync( f->dTxP, f->f.tx_pilots, U * f->f.n_pilots * 8, hipMemcpyHostToDevice, st ) );
         }
     }
-    for ( int ui = 0; ui < kServeUploadStreams; ui++ )
-        if ( used[ ui ] )
-        {
-            HIP_OR_THROW( hipEventRecord( D.uploaded[ ui ], S->ustream[ ui ] ) );
-            HIP_OR_THROW( hipStreamWaitEvent( st, D.uploaded[ ui ], 0 ) );
-        }
+    HIP_OR_THROW( hipEventRecord( D.uploaded, up ) );
+    HIP_OR_THROW( hipStreamWaitEvent( st, D.uploaded, 0 ) );
     const apsm::DetDesc* dd = (const apsm::DetDesc*)( base + detOff );
     for ( const DetL& d : detL )
     {
@@ -695,7 +697,7 @@ bool serverLaunchTrainRound( apsm_server* S )
     launchRoundPre( st, entry, base, entryOffs, combOff, nComb );
     if ( !sl.empty() )
     {
-        const int nside = std::min<int>( kBatchSideStreams, (int)sl.size() - 1 );
+        const int nside = std::min<int>( kServeSideStreams, (int)sl.size() - 1 );
         if ( nside > 0 )
             HIP_OR_THROW( hipEventRecord( S->fork, st ) );
         for ( int k = 0; k < nside; k++ )
