@@ -3,13 +3,14 @@
 // For one block (steps s_first .. s_last, entering rows [s_prev, s_last)):
 //   a0   = s_first - W(s_first): rows below a0 take no further projection in this block
 //          (frozen); rows in [a0, s_last) are the block's active rows.
-//   apsm_entry_mfma    (whole GPU) for every entering complex sample t and every user:
+// The first three run as roles of one launch (apsm_block_pre), the fourth after it:
+//   entry share        (whole GPU) for every entering complex sample t and every user:
 //          P_t = wL*thF.x_t + wG*sum_{j frozen, in the dictionary when t enters} c_j K(j,t),
 //          with thF = theta_0 + sum_{j < a0} c_j x_j.
-//   apsm_gram_mfma     kappa(t,j) = wL x_t.x_j + wG K(t,j) for every pair of active complex
+//   active Gram        kappa(t,j) = wL x_t.x_j + wG K(t,j) for every pair of active complex
 //          samples j <= t, as 3 floats per complex pair:
 //            A = kappa(a_t,a_j) = kappa(b_t,b_j), Bv = kappa(a_t,b_j), Cv = kappa(b_t,a_j).
-//   apsm_evict_corr    wG*sum_e c_e K(e,u) for the rows e leaving the dictionary at each
+//   eviction shares    wG*sum_e c_e K(e,u) for the rows e leaving the dictionary at each
 //          step and the window rows u that were already in the window.
 //   apsm_train_seq2    one workgroup per user runs the block's steps: f of an entering row
 //          is P_t + sum over active rows of c_j kappa(j,t); the sliding window Gram matrix
