@@ -324,37 +324,51 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
             int sl = sBase + tid;
             sl -= sl >= w2 ? w2 : 0;
             const int cv = sl >> 1, pt = sl & 1;
-            // parts beyond nParts hold zeros: the sum runs over PMAX without guards so the
-            // loads issue back to back
+            const int wpos = r - ( sRel - W );
+            const bool inWin = wpos >= 0, entering = r >= spRel; // entering implies inWin
+            const int rc = r < 0 ? 0 : r, e2 = entering ? r - spRel : 0; // e2 = 2*entering index + part
+            // every LDS read of the phase issued up front at clamped addresses (values a lane
+            // does not need are discarded), so the phase is one round trip, not a chain of them
+            float part[ PMAX ];
+#pragma unroll
+            for ( int t = 0; t < PMAX; t++ ) // parts beyond nParts hold zeros
+                part[ t ] = sRedF[ 2 * ( t * Wc + cv ) + pt ];
+            constexpr int kDotW0 = kDotBase / kWave;
+            float ent[ kSeqWaves - kDotW0 ];
+#pragma unroll
+            for ( int w = kDotW0; w < kSeqWaves; w++ )
+                ent[ w - kDotW0 ] = sRedE[ w * 2 * kNewMax + e2 ];
+            const float evv = pk[ pkRows + 2 * NEWPER + ( inWin ? wpos : 0 ) ];
+            const float pv = pk[ pkRows + e2 ];
+            const float fOld = sF[ sl ], yv = sY[ rc ], iv = sInv[ rc >> 1 ], cOld = sC[ rc ];
+            const float invW = 1.f / (float)W;
             float acc = 0.f;
 #pragma unroll
             for ( int t = 0; t < PMAX; t++ )
-                acc += sRedF[ 2 * ( t * Wc + cv ) + pt ];
-            const int wpos = r - ( sRel - W );
-            if ( a.hasEv && wpos >= 0 && r < spRel )
-                acc -= pk[ pkRows + 2 * NEWPER + wpos ];
-            float f = sF[ sl ] + acc;
+                acc += part[ t ];
+            if ( a.hasEv && inWin && !entering )
+                acc -= evv;
+            float f = fOld + acc;
             float beta = 0.f;
-            if ( wpos >= 0 )
+            if ( inWin )
             {
-                if ( r >= spRel )
+                if ( entering )
                 {
-                    const int e2 = r - spRel; // 2*entering index + part
-                    float e = pk[ pkRows + e2 ];
+                    float e = pv;
 #pragma unroll
-                    for ( int w = kDotBase / kWave; w < kSeqWaves; w++ )
-                        e += sRedE[ w * 2 * kNewMax + e2 ];
+                    for ( int w = 0; w < kSeqWaves - kDotW0; w++ )
+                        e += ent[ w ];
                     f = e;
                 }
-                const float res = sY[ r ] - f;
-                float b = 0.f;
+                const float res = yv - f;
+                float bb = 0.f;
                 if ( res > a.eB )
-                    b = res - a.eB;
+                    bb = res - a.eB;
                 else if ( res < -a.eB )
-                    b = res + a.eB;
-                b = b * ( 1.f / (float)W ) * sInv[ r >> 1 ];
-                beta = b;
-                sC[ r ] += b;
+                    bb = res + a.eB;
+                bb = bb * invW * iv;
+                beta = bb;
+                sC[ r ] = cOld + bb;
             }
             sF[ sl ] = f;
             sB[ sl ] = beta; // sl = 2*complex slot + part
