@@ -1655,7 +1655,8 @@ int apsm_server_create( const apsm_server_config* cfg, apsm_server** out )
         {
             S->cfg = *cfg;
             S->epoch = std::chrono::steady_clock::now();
-            // creation order: detect, upload, train, side (one hardware queue each)
+            // creation order: detect, upload, train, side (one hardware queue each while the
+            // process has no other streams)
             HIP_OR_THROW( hipStreamCreateWithFlags( &S->dstream, hipStreamNonBlocking ) );
             HIP_OR_THROW( hipStreamCreateWithFlags( &S->ustream, hipStreamNonBlocking ) );
             HIP_OR_THROW( hipStreamCreateWithFlags( &S->stream, hipStreamNonBlocking ) );
