@@ -44,12 +44,6 @@ __global__ void batch_stage( const StageDesc* __restrict__ d )
     stagePilotsBody( s.rx, s.tx, nullptr, s.N, s.U, s.S, s.Z, s.Zn, s.Y, s.C, s.ldYC, (int64_t)blockIdx.x * blockDim.x + threadIdx.x );
 }
 
-__global__ void batch_evict( const EvDesc* __restrict__ d )
-{
-    const EvDesc& e = d[ blockIdx.y ];
-    evictCorrBody( e.Z, e.Zn, e.C, e.EvF, e.a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x );
-}
-
 template <int TP, int NEWPER, int PMAX>
 __global__ void __launch_bounds__( kSeqThreads ) batch_seq2( const Seq2Desc* __restrict__ d )
 {

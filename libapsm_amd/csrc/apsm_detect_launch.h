@@ -127,20 +127,17 @@ hipError_t launchDetectMfma( int prec, int nmax, int umax, hipStream_t s, const 
                              const float* Zn, const float* C, int64_t ldC, const float* theta, int64_t G, int U, float wL,
                              float wG, float g2, float* est, int N );
 
-// Entry share of one block (chunks [chunk0, chunk0 + count) of nChunks), then the combine of the chunks and the
-// linear share into P [U][nNewC][2]; batched variants over n descriptors (gridX = most chunks,
-// resp. enough threads for the largest U * nNewC).
+// Entry share of one block (work items [chunk0, chunk0 + count) of nChunks * entryTiles(nNewC)),
+// then the combine of the chunks and the linear share into P [U][nNewC][2]; the batched combine
+// runs over n descriptors (gridX: enough threads for the largest U * nNewC).
 hipError_t launchEntryMfma( int nmax, int umax, hipStream_t s, const float* Z, const float* Zn, const float* C, float* part,
                             const EntryArgs& a, int64_t jBeg, int nChunks, int chunk0, int count );
 hipError_t launchEntryCombine( hipStream_t s, const float* Z, const float* thF, const float* part, float* P, const EntryArgs& a,
                                int nChunks );
-hipError_t launchBatchEntryMfma( int nmax, int umax, hipStream_t s, const EntryMfmaDesc* d, int n, int gridX );
 hipError_t launchBatchEntryCombine( hipStream_t s, const EntryMfmaDesc* d, int n, int gridX );
 
-hipError_t launchGramMfma( int nmax, hipStream_t s, const float* Z, const float* Zn, float* Gact, const GramArgs& a );
 hipError_t launchBlockPre( int nmax, int umax, hipStream_t s, const PreDesc& d );
 hipError_t launchBatchBlockPre( int nmax, int umax, hipStream_t s, const PreDesc* d, int n, int gridX );
-hipError_t launchBatchGramMfma( int nmax, hipStream_t s, const GramDesc* d, int n, int gridX );
 
 // Persistent batched detect: `grid` workgroups walk nItems work items of tiles of kMfmaSamples.
 hipError_t launchBatchDetectMfma( int prec, int nmax, int umax, hipStream_t s, const DetDesc* d, const WorkItem* items,

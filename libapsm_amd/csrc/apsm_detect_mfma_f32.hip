@@ -1,7 +1,6 @@
 // Matrix-core detect kernels with f32 cross-term inputs, every antenna and user bound.
 #include "apsm_detect_mfma.h"
 #include "apsm_entry_mfma.h"
-#include "apsm_gram_mfma.h"
 
 namespace apsm {
 
@@ -15,7 +14,7 @@ hipError_t launchEntryMfma( int nmax, int umax, hipStream_t s, const float* Z, c
 {
     if ( count <= 0 )
         return hipSuccess;
-    return launchEntryAnyN( nmax, umax, false, s, Z, Zn, C, part, a, jBeg, nChunks, nullptr, chunk0, count );
+    return launchEntryAnyN( nmax, umax, s, Z, Zn, C, part, a, jBeg, nChunks, chunk0, count );
 }
 
 hipError_t launchEntryCombine( hipStream_t s, const float* Z, const float* thF, const float* part, float* P, const EntryArgs& a,
@@ -26,22 +25,10 @@ hipError_t launchEntryCombine( hipStream_t s, const float* Z, const float* thF, 
     return hipGetLastError();
 }
 
-hipError_t launchBatchEntryMfma( int nmax, int umax, hipStream_t s, const EntryMfmaDesc* d, int n, int gridX )
-{
-    return launchEntryAnyN( nmax, umax, true, s, nullptr, nullptr, nullptr, nullptr, EntryArgs {}, 0, 0, d, n, gridX );
-}
-
 hipError_t launchBatchEntryCombine( hipStream_t s, const EntryMfmaDesc* d, int n, int gridX )
 {
     hipLaunchKernelGGL( batch_entry_combine, dim3( gridX, n ), dim3( 256 ), 0, s, d );
     return hipGetLastError();
-}
-
-hipError_t launchGramMfma( int nmax, hipStream_t s, const float* Z, const float* Zn, float* Gact, const GramArgs& a )
-{
-    if ( a.nA <= 0 )
-        return hipSuccess;
-    return launchGramAnyN( nmax, false, s, Z, Zn, Gact, a, nullptr, 0, 0 );
 }
 
 hipError_t launchBlockPre( int nmax, int umax, hipStream_t s, const PreDesc& d )
@@ -57,11 +44,6 @@ hipError_t launchBatchBlockPre( int nmax, int umax, hipStream_t s, const PreDesc
     if ( gridX <= 0 || n <= 0 )
         return hipSuccess;
     return launchPreAnyN( nmax, umax, true, s, d, n, gridX );
-}
-
-hipError_t launchBatchGramMfma( int nmax, hipStream_t s, const GramDesc* d, int n, int gridX )
-{
-    return launchGramAnyN( nmax, true, s, nullptr, nullptr, nullptr, GramArgs {}, d, n, gridX );
 }
 
 } // namespace apsm

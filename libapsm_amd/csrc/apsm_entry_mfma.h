@@ -197,24 +197,14 @@ __global__ void apsm_entry_combine( const float* __restrict__ Z, const float* __
     entryCombineBody( Z, thF, part, P, a, nChunks, blockIdx.x * blockDim.x + threadIdx.x );
 }
 
-template <int NMAX, int UMAX>
-__global__ void __launch_bounds__( kMfmaThreads ) batch_entry_mfma( const EntryMfmaDesc* __restrict__ d )
-{
-    const EntryMfmaDesc& e = d[ blockIdx.y ];
-    if ( (int)blockIdx.x >= e.nChunks * entryTiles( e.a.nNewC ) )
-        return;
-    entryChunkMfma<NMAX, UMAX>( e.Z, e.Zn, e.C, e.part, e.a, e.jBeg, e.nChunks, blockIdx.x );
-}
-
 __global__ void batch_entry_combine( const EntryMfmaDesc* __restrict__ d )
 {
     const EntryMfmaDesc& e = d[ blockIdx.y ];
     entryCombineBody( e.Z, e.thF, e.part, e.P, e.a, e.nChunks, blockIdx.x * blockDim.x + threadIdx.x );
 }
 
-// Single handle: work items [chunk0, chunk0 + gridX) of nChunks * entryTiles; batched: gridX =
-// most work items.
-// One block's precompute, workgroup b: entry work item, Gram blocks or eviction shares.
+// One block's precompute, workgroup b: entry work item (the first entryWg workgroups), Gram
+// blocks (the next gramWg) or eviction shares (the last evWg); batched: gridX = the largest sum.
 template <int NMAX, int UMAX>
 __device__ __forceinline__ void blockPreBody( const PreDesc& d, int b )
 {
@@ -289,43 +279,41 @@ inline hipError_t launchPreAnyN( int nmax, int umax, bool batch, hipStream_t s, 
     }
 }
 
+// Entry work items [chunk0, chunk0 + count) of one block of a single handle.
 template <int NMAX, int UMAX>
-hipError_t launchEntryNU( bool batch, hipStream_t s, const float* Z, const float* Zn, const float* C, float* part, const EntryArgs& a,
-                          int64_t jBeg, int nChunks, const EntryMfmaDesc* d, int n, int gridX )
+hipError_t launchEntryNU( hipStream_t s, const float* Z, const float* Zn, const float* C, float* part, const EntryArgs& a, int64_t jBeg,
+                          int nChunks, int chunk0, int count )
 {
-    if ( batch )
-        hipLaunchKernelGGL( ( batch_entry_mfma<NMAX, UMAX> ), dim3( gridX, n ), dim3( kMfmaThreads ), 0, s, d );
-    else
-        hipLaunchKernelGGL( ( apsm_entry_mfma<NMAX, UMAX> ), dim3( gridX ), dim3( kMfmaThreads ), 0, s, Z, Zn, C, part, a, jBeg, nChunks, n );
+    hipLaunchKernelGGL( ( apsm_entry_mfma<NMAX, UMAX> ), dim3( count ), dim3( kMfmaThreads ), 0, s, Z, Zn, C, part, a, jBeg, nChunks, chunk0 );
     return hipGetLastError();
 }
 
 template <int NMAX>
-hipError_t launchEntryN( int umax, bool batch, hipStream_t s, const float* Z, const float* Zn, const float* C, float* part,
-                         const EntryArgs& a, int64_t jBeg, int nChunks, const EntryMfmaDesc* d, int n, int gridX )
+hipError_t launchEntryN( int umax, hipStream_t s, const float* Z, const float* Zn, const float* C, float* part, const EntryArgs& a,
+                         int64_t jBeg, int nChunks, int chunk0, int count )
 {
     switch ( umax )
     {
-    case 1: return launchEntryNU<NMAX, 1>( batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
-    case 2: return launchEntryNU<NMAX, 2>( batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
-    case 4: return launchEntryNU<NMAX, 4>( batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
-    case 8: return launchEntryNU<NMAX, 8>( batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
-    case 12: return launchEntryNU<NMAX, 12>( batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
-    case 16: return launchEntryNU<NMAX, 16>( batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
+    case 1: return launchEntryNU<NMAX, 1>( s, Z, Zn, C, part, a, jBeg, nChunks, chunk0, count );
+    case 2: return launchEntryNU<NMAX, 2>( s, Z, Zn, C, part, a, jBeg, nChunks, chunk0, count );
+    case 4: return launchEntryNU<NMAX, 4>( s, Z, Zn, C, part, a, jBeg, nChunks, chunk0, count );
+    case 8: return launchEntryNU<NMAX, 8>( s, Z, Zn, C, part, a, jBeg, nChunks, chunk0, count );
+    case 12: return launchEntryNU<NMAX, 12>( s, Z, Zn, C, part, a, jBeg, nChunks, chunk0, count );
+    case 16: return launchEntryNU<NMAX, 16>( s, Z, Zn, C, part, a, jBeg, nChunks, chunk0, count );
     default: return hipErrorInvalidValue;
     }
 }
 
-inline hipError_t launchEntryAnyN( int nmax, int umax, bool batch, hipStream_t s, const float* Z, const float* Zn, const float* C,
-                                   float* part, const EntryArgs& a, int64_t jBeg, int nChunks, const EntryMfmaDesc* d, int n, int gridX )
+inline hipError_t launchEntryAnyN( int nmax, int umax, hipStream_t s, const float* Z, const float* Zn, const float* C, float* part,
+                                   const EntryArgs& a, int64_t jBeg, int nChunks, int chunk0, int count )
 {
     switch ( nmax )
     {
-    case 4: return launchEntryN<4>( umax, batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
-    case 8: return launchEntryN<8>( umax, batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
-    case 16: return launchEntryN<16>( umax, batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
-    case 32: return launchEntryN<32>( umax, batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
-    case 64: return launchEntryN<64>( umax, batch, s, Z, Zn, C, part, a, jBeg, nChunks, d, n, gridX );
+    case 4: return launchEntryN<4>( umax, s, Z, Zn, C, part, a, jBeg, nChunks, chunk0, count );
+    case 8: return launchEntryN<8>( umax, s, Z, Zn, C, part, a, jBeg, nChunks, chunk0, count );
+    case 16: return launchEntryN<16>( umax, s, Z, Zn, C, part, a, jBeg, nChunks, chunk0, count );
+    case 32: return launchEntryN<32>( umax, s, Z, Zn, C, part, a, jBeg, nChunks, chunk0, count );
+    case 64: return launchEntryN<64>( umax, s, Z, Zn, C, part, a, jBeg, nChunks, chunk0, count );
     default: return hipErrorInvalidValue;
     }
 }

@@ -74,49 +74,4 @@ __device__ __forceinline__ void gramBlockMfma( const float* __restrict__ Z, cons
     }
 }
 
-template <int NMAX>
-__global__ void __launch_bounds__( kMfmaThreads ) apsm_gram_mfma( const float* __restrict__ Z, const float* __restrict__ Zn,
-                                                                 float* __restrict__ Gact, const GramArgs a )
-{
-    const int64_t blk = (int64_t)blockIdx.x * ( kMfmaThreads / 64 ) + ( threadIdx.x >> 6 );
-    if ( blk < gramBlocks( a.nA ) )
-        gramBlockMfma<NMAX>( Z, Zn, Gact, a, blk );
-}
-
-template <int NMAX>
-__global__ void __launch_bounds__( kMfmaThreads ) batch_gram_mfma( const GramDesc* __restrict__ d )
-{
-    const GramDesc& g = d[ blockIdx.y ];
-    const int64_t blk = (int64_t)blockIdx.x * ( kMfmaThreads / 64 ) + ( threadIdx.x >> 6 );
-    if ( blk < gramBlocks( g.a.nA ) )
-        gramBlockMfma<NMAX>( g.Z, g.Zn, g.Gact, g.a, blk );
-}
-
-template <int NMAX>
-hipError_t launchGramN( bool batch, hipStream_t s, const float* Z, const float* Zn, float* Gact, const GramArgs& a, const GramDesc* d,
-                        int n, int gridX )
-{
-    constexpr int perWg = kMfmaThreads / 64;
-    if ( batch )
-        hipLaunchKernelGGL( ( batch_gram_mfma<NMAX> ), dim3( gridX, n ), dim3( kMfmaThreads ), 0, s, d );
-    else
-        hipLaunchKernelGGL( ( apsm_gram_mfma<NMAX> ), dim3( (unsigned)( ( gramBlocks( a.nA ) + perWg - 1 ) / perWg ) ), dim3( kMfmaThreads ), 0, s,
-                            Z, Zn, Gact, a );
-    return hipGetLastError();
-}
-
-inline hipError_t launchGramAnyN( int nmax, bool batch, hipStream_t s, const float* Z, const float* Zn, float* Gact, const GramArgs& a,
-                                  const GramDesc* d, int n, int gridX )
-{
-    switch ( nmax )
-    {
-    case 4: return launchGramN<4>( batch, s, Z, Zn, Gact, a, d, n, gridX );
-    case 8: return launchGramN<8>( batch, s, Z, Zn, Gact, a, d, n, gridX );
-    case 16: return launchGramN<16>( batch, s, Z, Zn, Gact, a, d, n, gridX );
-    case 32: return launchGramN<32>( batch, s, Z, Zn, Gact, a, d, n, gridX );
-    case 64: return launchGramN<64>( batch, s, Z, Zn, Gact, a, d, n, gridX );
-    default: return hipErrorInvalidValue;
-    }
-}
-
 } // namespace apsm

@@ -36,12 +36,6 @@ constexpr int kDotBase = 512;
 
 __host__ __device__ __forceinline__ int64_t triIdx( int64_t i ) { return i * ( i + 1 ) / 2; }
 
-__global__ void apsm_evict_corr( const float* __restrict__ Z, const float* __restrict__ Zn, const float* __restrict__ C,
-                                 float* __restrict__ EvF, const EvArgs a )
-{
-    evictCorrBody( Z, Zn, C, EvF, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x );
-}
-
 struct Seq2Args
 {
     int N, L, U, w2, Wc, q, nSteps, hasEv;
