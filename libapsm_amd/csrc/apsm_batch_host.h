@@ -516,8 +516,13 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
         {
             db.threads[ c ] = db.kernel < 0 ? kTuneThreads[ c ] : apsm::kMfmaThreads;
             const int tileSamples = db.kernel < 0 ? kTuneThreads[ c ] : apsm::kMfmaSamples;
+            // instances with the largest dictionaries first (a tile's cost is proportional to
+            // G): the workgroups of the persistent launch end on cheap tiles, not on a tail of
+            // expensive ones
+            std::vector<int> order( kv.second );
+            std::stable_sort( order.begin(), order.end(), [ & ]( int x, int y ) { return det[ x ].G > det[ y ].G; } );
             std::vector<apsm::WorkItem> items;
-            for ( int i : kv.second )
+            for ( int i : order )
                 for ( int64_t t = 0; t < ( det[ i ].M + tileSamples - 1 ) / tileSamples; t++ )
                     items.push_back( apsm::WorkItem { i, (int)t } );
             db.itemOff[ c ] = arena.add( items );
@@ -542,6 +547,7 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
     HIP_OR_THROW( hipStreamSynchronize( B->stream ) );
 
     // autotune: every (threads, workgroups per CU) candidate per bucket, best of two runs
+    std::vector<std::function<void( hipStream_t )>> detLaunches;
     hipEvent_t t0, t1;
     HIP_OR_THROW( hipEventCreate( &t0 ) );
     HIP_OR_THROW( hipEventCreate( &t1 ) );
@@ -549,11 +555,11 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
     {
         const apsm::DetDesc* dd = (const apsm::DetDesc*)( base + detOff );
         const int kernel = db.kernel, nmax = db.nmax, umax = db.umax;
-        auto launch = [ = ]( int threads, const apsm::WorkItem* it, int nItems, int grid ) {
+        auto launch = [ = ]( hipStream_t st, int threads, const apsm::WorkItem* it, int nItems, int grid ) {
             if ( kernel < 0 )
-                launchBatchDetectAny( B->stream, nmax, umax, threads, dd, it, nItems, grid );
+                launchBatchDetectAny( st, nmax, umax, threads, dd, it, nItems, grid );
             else
-                HIP_OR_THROW( apsm::launchBatchDetectMfma( kernel, nmax, umax, B->stream, dd, it, nItems, grid ) );
+                HIP_OR_THROW( apsm::launchBatchDetectMfma( kernel, nmax, umax, st, dd, it, nItems, grid ) );
         };
         float best = 1e30f;
         int bc = 0, bb = 4;
@@ -565,7 +571,7 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
                 for ( int rep = 0; rep < 2; rep++ )
                 {
                     HIP_OR_THROW( hipEventRecord( t0, B->stream ) );
-                    launch( db.threads[ c ], (const apsm::WorkItem*)( base + db.itemOff[ c ] ), db.nItems[ c ], grid );
+                    launch( B->stream, db.threads[ c ], (const apsm::WorkItem*)( base + db.itemOff[ c ] ), db.nItems[ c ], grid );
                     HIP_OR_THROW( hipEventRecord( t1, B->stream ) );
                     HIP_OR_THROW( hipEventSynchronize( t1 ) );
                     float t = 0.f;
@@ -582,9 +588,30 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
         const int threads = db.threads[ bc ], nItems = db.nItems[ bc ];
         const int grid = std::max( 1, std::min( nItems, B->cus * bb ) );
         const apsm::WorkItem* it = (const apsm::WorkItem*)( base + db.itemOff[ bc ] );
-        B->launches.push_back( [ = ]() { launch( threads, it, nItems, grid ); } );
+        detLaunches.push_back( [ = ]( hipStream_t st ) { launch( st, threads, it, nItems, grid ); } );
         B->tuning.push_back( { nmax, umax, threads, bb, nItems, kernel } );
     }
+    // the buckets run side by side (their tails overlap instead of draining the GPU in turn)
+    B->launches.push_back( [ B, detLaunches ]() {
+        const int nb = (int)detLaunches.size();
+        const int nside = std::min<int>( kBatchSideStreams, nb - 1 );
+        if ( nside > 0 )
+        {
+            HIP_OR_THROW( hipEventRecord( B->fork, B->stream ) );
+            for ( int k = 0; k < nside; k++ )
+                HIP_OR_THROW( hipStreamWaitEvent( B->side[ k ], B->fork, 0 ) );
+        }
+        for ( int b = 0; b < nb; b++ )
+        {
+            const int lane = b % ( nside + 1 );
+            detLaunches[ b ]( lane == 0 ? B->stream : B->side[ lane - 1 ] );
+        }
+        for ( int k = 0; k < nside; k++ )
+        {
+            HIP_OR_THROW( hipEventRecord( B->join[ k ], B->side[ k ] ) );
+            HIP_OR_THROW( hipStreamWaitEvent( B->stream, B->join[ k ], 0 ) );
+        }
+    } );
     (void)hipEventDestroy( t0 );
     (void)hipEventDestroy( t1 );
 

@@ -227,7 +227,7 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     // 8..15 take the active rows' share of each entering f (the c of rows that entered up to
     // step ks - 1 are final for it), waves 5..7 stage the Gram column of each entering sample.
     // It runs beside the Gram product of step ks - 1.
-    auto stage = [ & ]( int ks ) {
+    auto stage = [ & ]( int ks, int pc0 ) { // pc0: complex slot of the step's first entering sample
         const float* pk = sPk + ( ks % kPkBufs ) * kPkCap;
         const int c0 = a.newRel0 + ks * NEWPER; // first entering complex sample
         if ( wave >= kDotBase / kWave )
@@ -268,7 +268,7 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
             // the last entering sample; kappa(k, r) comes from the package row of max(k, r) at
             // the position of min(k, r)
             const int kRef = c0 + NEWPER - 1;
-            const int pRef = ( cMod + kRef ) % Wc;
+            const int pRef = pc0 + NEWPER - 1 < Wc ? pc0 + NEWPER - 1 : pc0 + NEWPER - 1 - Wc;
             const int ct = tid - kColBase; // t-slot staged by this thread
             if ( ct < a.nParts * TP )
             {
@@ -294,9 +294,12 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
         }
     };
 
+    // slots advance by a fixed amount per step: tracked incrementally (no division in the loop)
+    int sBase = ( a.a0modW + a.sFirstRel ) % w2; // slot of row s (== slot of row s - w2)
+    int pC0 = ( cMod + a.newRel0 ) % Wc; // complex slot of the step's first entering sample
     waitVm<0>();
     __syncthreads();
-    stage( 0 );
+    stage( 0, pC0 );
     ldsBarrier();
 
     for ( int kk = 0; kk < a.nSteps; kk++ )
@@ -306,7 +309,7 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
         const int spRel = sRel - a.q;
         const int W = a.ramp ? ( sRel < w2 ? sRel : w2 ) : w2;
         const int c0 = a.newRel0 + kk * NEWPER; // first entering complex sample
-        const int sBase = ( a.a0modW + sRel ) % w2; // slot of row s (== slot of row s - w2)
+        const int pNext = pC0 + NEWPER < Wc ? pC0 + NEWPER : pC0 + NEWPER - Wc;
 
         // 1. projection over the window. Thread tid < w2 takes row r = s - w2 + tid (the last
         //    w2 rows cover every slot once): its f gets the previous step's Gram product and,
@@ -372,12 +375,12 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
         //    takes the column staged beside the previous step's product
         {
             const int kRef = c0 + NEWPER - 1;
-            const int pRef = ( cMod + kRef ) % Wc;
+            const int pRef = pC0 + NEWPER - 1 < Wc ? pC0 + NEWPER - 1 : pC0 + NEWPER - 1 - Wc;
 #pragma unroll
             for ( int i = 0; i < NEWPER; i++ )
             {
                 const int k = c0 + i;
-                const int p = ( cMod + k ) % Wc;
+                const int p = pC0 + i < Wc ? pC0 + i : pC0 + i - Wc;
                 const int jj = p - tBase;
                 if ( gThread && jj >= 0 && jj < TP )
                 {
@@ -411,7 +414,7 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
 #pragma unroll
             for ( int i = 0; i < NEWPER; i++ )
             {
-                const int p = ( cMod + c0 + i ) % Wc;
+                const int p = pC0 + i < Wc ? pC0 + i : pC0 + i - Wc;
                 if ( gThread && v == p )
                 {
                     const float4* col = reinterpret_cast<const float4*>( sCol + i * 3 * kColStride + part * TPA );
@@ -440,7 +443,7 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
         waitVmN( kk + 2 < a.nSteps ? myChunks : 0 );
         ldsBarrier(); // B
         if ( kk + 1 < a.nSteps )
-            stage( kk + 1 );
+            stage( kk + 1, pNext );
         // 3. partial f_v += sum_t kappa(t, v) beta_t from the register Gram (summed over the
         //    t-slot parts at the start of the next step)
         if ( gThread )
@@ -462,6 +465,9 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
         // the package kPkBufs - 1 steps ahead goes into the buffer read up to the previous step
         if ( kk + kPkBufs - 1 < a.nSteps )
             pkIssue( kk + kPkBufs - 1 );
+        sBase += a.q;
+        sBase -= sBase >= w2 ? w2 : 0;
+        pC0 = pNext;
         ldsBarrier(); // C
     }
     if ( tid < w2 )
