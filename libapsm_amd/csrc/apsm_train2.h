@@ -230,7 +230,20 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     auto stage = [ & ]( int ks, int pc0 ) { // pc0: complex slot of the step's first entering sample
         const float* pk = sPk + ( ks % kPkBufs ) * kPkCap;
         const int c0 = a.newRel0 + ks * NEWPER; // first entering complex sample
-        if ( wave >= kDotBase / kWave )
+        if ( wave >= kDotBase / kWave && wave * kWave - kDotBase >= c0 )
+        {
+            // no active row in this wave's range (early steps of a block): its partials are 0
+            if ( lane == 0 )
+            {
+#pragma unroll
+                for ( int i = 0; i < NEWPER; i++ )
+                {
+                    sRedE[ wave * 2 * kNewMax + 2 * i ] = 0.f;
+                    sRedE[ wave * 2 * kNewMax + 2 * i + 1 ] = 0.f;
+                }
+            }
+        }
+        else if ( wave >= kDotBase / kWave )
         {
             float ea[ NEWPER ], eb[ NEWPER ];
 #pragma unroll
