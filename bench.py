@@ -27,6 +27,14 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3
 
 
+# HBM bytes of one train call of a workload (all its dispatches: sequencer, block precompute,
+# combine, staging, theta), from two rocprofv3 --pmc passes over `bench.py --steps 1 --warmup 0
+# --no-cpu-baseline --no-lls --no-batched --no-mixed --no-serving`: FETCH_SIZE 6 586 792 KiB
+# x 2.000 (a 1 GiB device copy, `bench.py --only copy`, reads 524 301 KiB of FETCH_SIZE) +
+# WRITE_SIZE 4 601 809 KiB x 1.000 (the copy reads 1 048 576 KiB).
+TRAIN_HBM_BYTES = {"plain-16ant-16qam": (6586792 * 2.000 + 4601809 * 1.000) * 1024}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,8 +50,9 @@ def parse():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--serving-frames", type=int, default=600)
     ap.add_argument("--serving-links", type=int, default=2048)
-    ap.add_argument("--only", choices=["batched", "mixed", "serving"], default=None,
-                    help="run one further section alone (for profiling); prints {section: result}")
+    ap.add_argument("--only", choices=["batched", "mixed", "serving", "copy"], default=None,
+                    help="run one further section alone (for profiling); prints {section: result}; "
+                         "copy: a device-to-device copy of known size (calibrates HBM byte counters)")
     ap.add_argument("--mixed-samples", type=int, default=None, help="override the mixed workload's detect samples")
     return ap.parse_args()
 
@@ -297,6 +306,23 @@ def bench_mixed(steps, samples=None):
     return out
 
 
+def bench_copy(n_bytes=1 << 30, reps=4):
+    """Device-to-device copies of n_bytes (torch's copy kernel): the reference reading of the HBM
+    byte counters (FETCH_SIZE, WRITE_SIZE) for a kernel that reads and writes n_bytes once."""
+    import torch
+    a = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / reps
+    return {"bytes_read": n_bytes, "bytes_written": n_bytes, "launches": reps + 1, "ms": ms,
+            "gb_per_s": 2.0 * n_bytes / (ms / 1000.0) / 1e9}
+
+
 def bench_serving(n_frames=600, n_links=2048, rates=(1000.0, 2000.0, 4000.0, 8000.0)):
     """Serving loop: up to n_links links (8..32 antennas, QPSK or 16QAM, windowSize 40,
     dictionary 1024) admitted by an HBM budget; frames of 14 OFDM symbols x 12 x U[1, 273]
@@ -415,6 +441,8 @@ def main():
             res = bench_batched(args.batched_instances, max(1, args.steps))
         elif args.only == "mixed":
             res = bench_mixed(max(1, args.steps), args.mixed_samples)
+        elif args.only == "copy":
+            res = bench_copy()
         else:
             res = bench_serving(args.serving_frames, args.serving_links)
         print(json.dumps({args.only: res}))
@@ -482,7 +510,8 @@ def main():
     dom = max(kern, key=lambda k: kern[k]["ms"])
     ach = kern[dom]["flops"] / (kern[dom]["ms"] / 1000.0) / 1e12
     roofline = {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_FP32_TFLOPS, "traffic": None}
+                "frac": ach / PEAK_FP32_TFLOPS,
+                "traffic": TRAIN_HBM_BYTES.get(args.workload) if dom == "apsm_train_seq" else None}
     other = "apsm_detect_kernel" if dom == "apsm_train_seq" else "apsm_train_seq"
     ach2 = kern[other]["flops"] / (kern[other]["ms"] / 1000.0) / 1e12
 
