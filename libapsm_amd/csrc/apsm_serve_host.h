@@ -551,8 +551,11 @@ bool serverLaunchDetect( apsm_server* S )
     {
         const int kernel = kv.first[ 0 ];
         const int tile = kernel < 0 ? apsm::kDetThreads : apsm::kMfmaSamples;
+        // frames of the largest dictionaries first (see the batched detect)
+        std::vector<int> order( kv.second );
+        std::stable_sort( order.begin(), order.end(), [ & ]( int x, int y ) { return det[ x ].G > det[ y ].G; } );
         std::vector<apsm::WorkItem> items;
-        for ( int i : kv.second )
+        for ( int i : order )
             for ( int64_t t = 0; t < ( det[ i ].M + tile - 1 ) / tile; t++ )
                 items.push_back( apsm::WorkItem { i, (int)t } );
         detL.push_back( DetL { kernel, kv.first[ 1 ], kv.first[ 2 ], (int)items.size(), arena.add( items ) } );
