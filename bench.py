@@ -137,15 +137,11 @@ def bench_batched(count, steps):
 
     inst = datagen.batched_instances(count)
 
-    def gen(x):
-        return datagen.generate(x["N"], x["U_tx"], x["T"], x["M"], modulation=x["modulation"], snr_db=x["snr_db"], seed=x["seed"])
-
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        data = list(ex.map(gen, inst))
+        data = list(ex.map(datagen.batched_payload, inst))
     t_gen = time.perf_counter() - t0
-    cfgs = [{"windowSize": x["window"], "dictionarySize": x["dict_size"], "eB": x["eB"],
-             "gaussianKernelVariance": datagen.kernel_variance_for_sigma(x["sigma"], x["N"])} for x in inst]
+    cfgs = [datagen.batched_config(x) for x in inst]
     up = lambda z: DeviceArray(np.ascontiguousarray(z, dtype=np.complex64).view(np.float32))  # noqa: E731
     d_rxt = [up(d["rx_train"]) for d in data]
     d_txt = [up(d["tx_train"][x["user"]][None, :]) for d, x in zip(data, inst)]

@@ -277,10 +277,14 @@ apsm_handle* apsm_server_link(apsm_server* s, uint32_t link);
 /* Queues a frame (arrival = now); *ticket identifies its result. */
 int apsm_server_submit(apsm_server* s, const apsm_frame* frame, uint64_t* ticket);
 /* Makes progress (retires finished rounds, launches the next) and reports finished frames,
- * up to cap; wait != 0 keeps going until every submitted frame has finished. */
+ * up to cap; wait != 0 keeps going until every submitted frame has finished. A reported
+ * frame's est is complete in host memory (pinned or pageable) when the call returns. On a
+ * failure every frame still held is dropped (not reported) and the server is idle again. */
 int apsm_server_poll(apsm_server* s, int wait, apsm_frame_result* out, uint32_t cap, uint32_t* n);
 /* Serves n frames arriving at frames[i].arrival_s (non-decreasing) after the call starts, on
- * the host's clock, and returns when all have finished; results[i] belongs to frames[i]. */
+ * the host's clock, and returns when all have finished; results[i] belongs to frames[i].
+ * Every frame is checked before any is enqueued (APSM_ERR_ARG: nothing ran); on a failure
+ * during the run the frames are dropped and the server is idle again. */
 int apsm_server_replay(apsm_server* s, const apsm_frame* frames, uint32_t n, apsm_frame_result* results);
 /* Rounds launched so far, the largest number of frames one round carried, device bytes of
  * the links. */

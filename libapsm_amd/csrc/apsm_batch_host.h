@@ -206,6 +206,11 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
                               "batched instances need the blocked train path, a configured kernel variance, no LLS initialisation, "
                               "no shuffle and at most 16 users" };
     }
+    // a run enqueued before this call may still read the descriptors, graphs and instance
+    // buffers replaced below (the descriptor upload is not ordered after the batch streams)
+    HIP_OR_THROW( hipStreamSynchronize( B->stream ) );
+    for ( int k = 0; k < kBatchSideStreams; k++ )
+        HIP_OR_THROW( hipStreamSynchronize( B->side[ k ] ) );
     for ( int g = 0; g < 2; g++ )
     {
         if ( B->exec[ g ] )

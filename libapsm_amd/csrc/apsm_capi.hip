@@ -1533,8 +1533,9 @@ int apsm_batch_create( uint32_t n, const apsm_params* params, const uint32_t* nu
             for ( uint32_t i = 0; i < n; i++ )
             {
                 apsm_handle* h = nullptr;
-                if ( apsm_create( &params[ i ], num_antennas[ i ], num_users[ i ], &h ) != APSM_OK )
-                    throw ApsmError { APSM_ERR_ARG, std::string( "instance " ) + std::to_string( i ) + ": " + g_lastError };
+                const int rc = apsm_create( &params[ i ], num_antennas[ i ], num_users[ i ], &h );
+                if ( rc != APSM_OK )
+                    throw ApsmError { rc, std::string( "instance " ) + std::to_string( i ) + ": " + g_lastError };
                 B->inst.push_back( h );
             }
         }
@@ -1744,8 +1745,9 @@ int apsm_server_add_link( apsm_server* S, const apsm_params* p, uint32_t num_ant
         ARG_CHECK( S && p && link, "null argument" );
         ARG_CHECK( max_pilots >= 1, "max_pilots must be positive" );
         apsm_handle* h = nullptr;
-        if ( apsm_create( p, num_antennas, num_users, &h ) != APSM_OK )
-            throw ApsmError { APSM_ERR_ARG, g_lastError };
+        const int rc = apsm_create( p, num_antennas, num_users, &h );
+        if ( rc != APSM_OK )
+            throw ApsmError { rc, g_lastError };
         try
         {
             checkServable( h );
@@ -1790,11 +1792,21 @@ int apsm_server_poll( apsm_server* S, int wait, apsm_frame_result* out, uint32_t
 {
     return guarded( [ & ] {
         ARG_CHECK( S && n, "null argument" );
-        do
-            serverProgress( S );
-        while ( wait && !serverIdle( S ) );
-        if ( wait )
-            HIP_OR_THROW( hipStreamSynchronize( S->dstream ) ); // host-side ends of pageable copies
+        try
+        {
+            do
+                serverProgress( S );
+            while ( wait && !serverIdle( S ) );
+        }
+        catch ( ... )
+        {
+            serverAbort( S ); // the frames in flight are dropped; the server is idle again
+            throw;
+        }
+        // host-side ends of the estimate copies into pageable buffers: the detect event of a
+        // batch completes before the runtime has finished such a copy on the host
+        if ( wait || ( out && cap > 0 && !S->finished.empty() ) )
+            HIP_OR_THROW( hipStreamSynchronize( S->dstream ) );
         uint32_t k = 0;
         while ( k < cap && !S->finished.empty() && out )
         {
@@ -1814,6 +1826,8 @@ int apsm_server_replay( apsm_server* S, const apsm_frame* frames, uint32_t n, ap
         ARG_CHECK( serverIdle( S ) && S->finished.empty(), "replay needs an idle server" );
         for ( uint32_t i = 1; i < n; i++ )
             ARG_CHECK( frames[ i ].arrival_s >= frames[ i - 1 ].arrival_s, "replay frames must be in arrival order" );
+        for ( uint32_t i = 0; i < n; i++ ) // a bad frame fails the call before any work is enqueued
+            serverCheckFrame( S, frames[ i ] );
         const uint64_t ticket0 = S->nextTicket;
         S->nextTicket += n;
         const double t0 = serverNow( S );
@@ -1841,8 +1855,9 @@ int apsm_server_replay( apsm_server* S, const apsm_frame* frames, uint32_t n, ap
         }
         catch ( ... )
         {
-            (void)hipStreamSynchronize( S->stream );
-            (void)hipStreamSynchronize( S->dstream );
+            // every frame of this replay is dropped, so no later call touches the caller's
+            // buffers, and the server is idle again
+            serverAbort( S );
             throw;
         }
     } );

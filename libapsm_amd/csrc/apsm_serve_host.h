@@ -741,23 +741,25 @@ void serverProgress( apsm_server* S )
     serverLaunchTrainRound( S );
 }
 
-void serverEnqueue( apsm_server* S, const apsm_frame& f, double arrival, uint64_t ticket )
+// Throws APSM_ERR_ARG for a frame the server cannot take; enqueues nothing.
+void serverCheckFrame( const apsm_server* S, const apsm_frame& f )
 {
     ARG_CHECK( f.link < S->links.size() && S->links[ f.link ].h, "unknown link" );
     const ServeLink& L = S->links[ f.link ];
     ARG_CHECK( f.rx_data && f.est && f.n_data > 0, "frame without data samples" );
     ARG_CHECK( f.n_pilots == 0 || ( f.rx_pilots && f.tx_pilots ), "frame pilots missing" );
     ARG_CHECK( f.n_pilots <= L.maxPilots, "frame has more pilots than the link reserved" );
+    ARG_CHECK( frameBytes( L, f ) <= S->cfg.frame_arena, "frame larger than the frame arena" );
+}
+
+void serverEnqueue( apsm_server* S, const apsm_frame& f, double arrival, uint64_t ticket )
+{
+    serverCheckFrame( S, f );
     auto* sf = new ServeFrame;
     sf->f = f;
     sf->ticket = ticket;
     sf->arrival = arrival;
-    sf->arenaBytes = frameBytes( L, f );
-    if ( sf->arenaBytes > S->cfg.frame_arena )
-    {
-        delete sf;
-        throw ApsmError { APSM_ERR_ARG, "frame larger than the frame arena" };
-    }
+    sf->arenaBytes = frameBytes( S->links[ f.link ], f );
     S->waiting.push_back( sf );
 }
 
@@ -802,6 +804,25 @@ void serverDropFrames( apsm_server* S )
     S->pendingTrain.clear();
     S->training.clear();
     S->finished.clear();
+}
+
+// After a failure inside the serving loop: waits for everything enqueued on the server's
+// streams, then drops every frame it holds (their host buffers are no longer touched) and
+// returns the server to idle (arena free, no link busy, nothing in flight).
+void serverAbort( apsm_server* S )
+{
+    for ( hipStream_t st : { S->stream, S->dstream, S->ustream } )
+        if ( st )
+            (void)hipStreamSynchronize( st );
+    for ( hipStream_t st : S->side )
+        if ( st )
+            (void)hipStreamSynchronize( st );
+    serverDropFrames( S );
+    S->roundsInFlight.clear();
+    S->detectsInFlight.clear();
+    S->arenaAlloc.reset( (size_t)S->cfg.frame_arena );
+    for ( ServeLink& L : S->links )
+        L.busy = false;
 }
 
 } // namespace

@@ -160,6 +160,18 @@ def batched_instances(count=512, seed=3):
     return out
 
 
+def batched_payload(x):
+    """Samples of one batched instance (a draw of batched_instances): pilots first, then data,
+    from the instance's seed."""
+    return generate(x["N"], x["U_tx"], x["T"], x["M"], modulation=x["modulation"], snr_db=x["snr_db"], seed=x["seed"])
+
+
+def batched_config(x):
+    """Detector configuration of one batched instance."""
+    return {"windowSize": x["window"], "dictionarySize": x["dict_size"], "eB": x["eB"],
+            "gaussianKernelVariance": kernel_variance_for_sigma(x["sigma"], x["N"])}
+
+
 def serving_links(links=2048, seed=5):
     """Per-link parameters of the serving workload: 8..32 antennas, QPSK or 16QAM."""
     rng = np.random.Generator(np.random.PCG64(seed))

@@ -605,6 +605,233 @@ uint32_t demap1d( double x, uint32_t ns )
     return xi ^ ( xi >> 1 );
 }
 
+// ---------------------------------------------------------------------------------------
+// Streamed multi-user form (FP64): the incremental form above for U detectors that share
+// one received pilot stream (one target row per user), with the window Gram matrix kept
+// between steps. Per step only the Gram entries of the entering rows are new (O(W*L)
+// kernel evaluations instead of O(W^2*L)), and the kernel values of an entering row
+// against the dictionary are computed once for all users. Every value is computed with
+// the same expressions, in the same order, as incrementalStep / evalModel, so each user's
+// result equals the single-user incremental form (and, through it, the naive reference
+// schedule of apsm_train.cu:101-160,333-428). It exists so that workloads of 10^5 steps
+// (200k pilots, window 160, dictionary 4096, 10 users) finish in minutes on host cores.
+// Scope: trainPasses 1, normConstraint 0, no LLS initialisation, no shuffle.
+// ---------------------------------------------------------------------------------------
+struct Multi
+{
+    OrcParams p;
+    int N = 0, L = 0, U = 0;
+    int64_t H = 0, w2 = 0;
+    double wL = 0, wG = 0, var = 0;
+    bool varSet = false;
+    std::vector<double> X; // every row ever entered [row][L]
+    std::vector<double> y; // [row][U]
+    std::vector<double> c; // [row][U]
+    std::vector<double> theta; // [U][L]
+    std::vector<double> F; // [slot][U], slot = row mod w2 (rows of the window)
+    std::vector<double> Kw; // [slot][slot] window Gram (wL x.x + wG g)
+    int64_t used = 0;
+
+    int64_t dictLo( int64_t s ) const { return std::max<int64_t>( 0, s - H ); }
+    const double* row( int64_t r ) const { return &X[ r * L ]; }
+    int64_t slot( int64_t r ) const { return r % w2; }
+};
+
+double multiKappa( const Multi& m, int64_t i, int64_t j )
+{
+    const double* a = m.row( i );
+    const double* b = m.row( j );
+    return m.wL * dot( a, b, m.L ) + m.wG * gauss( a, b, m.L, m.var );
+}
+
+void multiStep( Multi& m, int64_t sPrev, int64_t s, int64_t W )
+{
+    const int L = m.L, U = m.U;
+    const int64_t t0 = s - W;
+    const int64_t loPrev = m.dictLo( sPrev ), lo = m.dictLo( s );
+    // evictions: window rows that stay lose the evicted rows' Gaussian share
+    for ( int64_t e = loPrev; e < lo; e++ )
+    {
+#pragma omp parallel for schedule( static )
+        for ( int64_t u = t0; u < sPrev; u++ )
+        {
+            double g = -1.0;
+            for ( int k = 0; k < U; k++ )
+            {
+                const double ce = m.c[ e * U + k ];
+                if ( ce == 0.0 )
+                    continue;
+                if ( g < 0.0 )
+                    g = gauss( m.row( e ), m.row( u ), L, m.var );
+                m.F[ m.slot( u ) * U + k ] -= m.wG * ce * g;
+            }
+        }
+    }
+    // entering rows: f against the dictionary [lo, sPrev) under the current state
+    const int64_t nd = sPrev - lo;
+    std::vector<double> g( std::max<int64_t>( nd, 1 ) );
+    for ( int64_t t = std::max( sPrev, t0 ); t < s; t++ ) // rows below t0 never enter the window
+    {
+        const double* xt = m.row( t );
+#pragma omp parallel for schedule( static )
+        for ( int64_t j = 0; j < nd; j++ )
+            g[ j ] = gauss( m.row( lo + j ), xt, L, m.var );
+#pragma omp parallel for schedule( static )
+        for ( int k = 0; k < U; k++ )
+        {
+            double lin = m.wL * dot( &m.theta[ (size_t)k * L ], xt, L );
+            double gs = 0;
+            for ( int64_t j = 0; j < nd; j++ )
+            {
+                const double cj = m.c[ ( lo + j ) * U + k ];
+                if ( cj == 0.0 )
+                    continue;
+                gs += m.wG * g[ j ] * cj;
+            }
+            m.F[ m.slot( t ) * U + k ] = lin + gs;
+        }
+    }
+    // Gram entries of the entering rows against the window
+#pragma omp parallel for schedule( static )
+    for ( int64_t v = t0; v < s; v++ )
+        for ( int64_t t = std::max( sPrev, t0 ); t < s; t++ )
+        {
+            const double kv = multiKappa( m, t, v );
+            m.Kw[ m.slot( t ) * m.w2 + m.slot( v ) ] = kv;
+            m.Kw[ m.slot( v ) * m.w2 + m.slot( t ) ] = kv;
+        }
+    // projections of every window row under the pre-step state, per user
+    std::vector<double> beta( (size_t)W * U );
+#pragma omp parallel for schedule( static )
+    for ( int k = 0; k < U; k++ )
+    {
+        for ( int64_t b = 0; b < W; b++ )
+        {
+            const int64_t t = t0 + b;
+            const double* xt = m.row( t );
+            double r = m.y[ t * U + k ] - m.F[ m.slot( t ) * U + k ];
+            double w = softThreshold( r, m.p.eB );
+            w /= double( W );
+            w /= ( m.wL * dot( xt, xt, L ) + m.wG );
+            beta[ (size_t)k * W + b ] = w;
+        }
+        for ( int64_t b = 0; b < W; b++ )
+        {
+            const int64_t t = t0 + b;
+            const double bb = beta[ (size_t)k * W + b ];
+            m.c[ t * U + k ] += bb;
+            for ( int d = 0; d < L; d++ )
+                m.theta[ (size_t)k * L + d ] += bb * m.X[ t * L + d ];
+        }
+    }
+    // window rows absorb the update through the window Gram matrix
+#pragma omp parallel for collapse( 2 ) schedule( static )
+    for ( int k = 0; k < U; k++ )
+        for ( int64_t u = t0; u < s; u++ )
+        {
+            const double* kr = &m.Kw[ m.slot( u ) * m.w2 ];
+            const double* bk = &beta[ (size_t)k * W ];
+            double acc = 0;
+            for ( int64_t b = 0; b < W; b++ )
+                if ( bk[ b ] != 0.0 )
+                    acc += bk[ b ] * kr[ m.slot( t0 + b ) ];
+            m.F[ m.slot( u ) * U + k ] += acc;
+        }
+}
+
+int multiTrain( Multi& m, const float* rx, const float* tx, int64_t T )
+{
+    const OrcParams& p = m.p;
+    const int N = m.N, L = m.L, U = m.U;
+    const int64_t q = p.sample_step;
+    if ( q <= 0 || ( q % 2 ) != 0 || p.window_size <= 0 || p.window_size > p.dictionary_size )
+    {
+        g_err = "sampleStep must be a positive even number and windowSize in [1, dictionarySize]";
+        return -1;
+    }
+    if ( p.train_passes > 1 || p.norm_constraint > 0 || p.lls_initialization || p.training_shuffle )
+    {
+        g_err = "the streamed form needs trainPasses 1, no normConstraint, no LLS initialisation, no shuffle";
+        return -1;
+    }
+    const int64_t base = m.used;
+    if ( !m.varSet )
+    {
+        if ( p.gaussian_kernel_variance != 0 )
+            m.var = double( p.gaussian_kernel_variance ) * double( N * N );
+        else
+            m.var = double( varianceHeuristic( rx, N, T, shufflePermutation( T, false ) ) );
+        m.varSet = true;
+    }
+    const int64_t R2 = 2 * T;
+    m.X.resize( ( base + R2 ) * L );
+    m.y.resize( ( base + R2 ) * U );
+    m.c.resize( ( base + R2 ) * U, 0.0 );
+    std::vector<double> a( L ), b( L );
+    for ( int64_t k = 0; k < T; k++ )
+    {
+        embed<double>( rx, N, T, k, &m.X[ ( base + 2 * k ) * L ], &m.X[ ( base + 2 * k + 1 ) * L ] );
+        for ( int u = 0; u < U; u++ )
+        {
+            m.y[ ( base + 2 * k ) * U + u ] = tx[ ( (int64_t)u * T + k ) * 2 ];
+            m.y[ ( base + 2 * k + 1 ) * U + u ] = tx[ ( (int64_t)u * T + k ) * 2 + 1 ];
+        }
+    }
+    const bool fullStart = p.start_with_full_window && base == 0;
+    const int64_t first = fullStart ? m.w2 : q;
+    int64_t sPrev = base;
+    for ( int64_t sl = first; sl <= R2; sl += q )
+    {
+        const int64_t s = base + sl;
+        m.used = s;
+        multiStep( m, sPrev, s, std::min( m.w2, s ) );
+        sPrev = s;
+    }
+    return 0;
+}
+
+// est [U][M][2]
+void multiDetect( const Multi& m, const float* rx, int64_t M, double* est )
+{
+    const int N = m.N, L = m.L, U = m.U;
+    const int64_t lo = m.dictLo( m.used ), hi = m.used;
+#pragma omp parallel
+    {
+        std::vector<double> a( L ), b( L ), ga( U ), gb( U );
+#pragma omp for schedule( dynamic, 64 )
+        for ( int64_t k = 0; k < M; k++ )
+        {
+            embed<double>( rx, N, M, k, a.data(), b.data() );
+            std::fill( ga.begin(), ga.end(), 0.0 );
+            std::fill( gb.begin(), gb.end(), 0.0 );
+            for ( int64_t j = lo; j < hi; j++ )
+            {
+                const double* xj = m.row( j );
+                double ea = -1.0, eb = -1.0;
+                for ( int u = 0; u < U; u++ )
+                {
+                    const double cj = m.c[ j * U + u ];
+                    if ( cj == 0.0 )
+                        continue;
+                    if ( ea < 0.0 )
+                    {
+                        ea = gauss( xj, a.data(), L, m.var );
+                        eb = gauss( xj, b.data(), L, m.var );
+                    }
+                    ga[ u ] += m.wG * ea * cj;
+                    gb[ u ] += m.wG * eb * cj;
+                }
+            }
+            for ( int u = 0; u < U; u++ )
+            {
+                const double* th = &m.theta[ (size_t)u * L ];
+                est[ ( (int64_t)u * M + k ) * 2 ] = m.wL * dot( th, a.data(), L ) + ga[ u ];
+                est[ ( (int64_t)u * M + k ) * 2 + 1 ] = m.wL * dot( th, b.data(), L ) + gb[ u ];
+            }
+        }
+    }
+}
+
 } // namespace
 
 extern "C" {
@@ -724,6 +951,63 @@ int orc_apsm_set_state( void* h, const double* rows, const double* coeffs, const
     m->varSet = true;
     m->anyCall = true;
     return 0;
+}
+
+// ---- FP64 oracle, streamed multi-user form ----
+void* orc_multi_create( const OrcParams* p, int N, int U )
+{
+    if ( N <= 0 || U <= 0 || p->dictionary_size <= 0 )
+    {
+        g_err = "numAntennas, users and dictionarySize must be positive";
+        return nullptr;
+    }
+    Multi* m = new Multi();
+    m->p = *p;
+    m->N = N;
+    m->L = 2 * N;
+    m->U = U;
+    m->H = 2 * (int64_t)p->dictionary_size;
+    m->w2 = 2 * (int64_t)p->window_size;
+    m->wG = p->gaussian_kernel_weight;
+    m->wL = p->linear_kernel_weight;
+    m->theta.assign( (size_t)U * m->L, 0.0 );
+    m->F.assign( (size_t)std::max<int64_t>( m->w2, 1 ) * U, 0.0 );
+    m->Kw.assign( (size_t)std::max<int64_t>( m->w2, 1 ) * std::max<int64_t>( m->w2, 1 ), 0.0 );
+    return m;
+}
+
+void orc_multi_destroy( void* h ) { delete static_cast<Multi*>( h ); }
+
+// rx [N][T][2], tx [U][T][2]
+int orc_multi_train( void* h, const float* rx, const float* tx, int64_t T )
+{
+    try
+    {
+        return multiTrain( *static_cast<Multi*>( h ), rx, tx, T );
+    }
+    catch ( const std::exception& e )
+    {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// est [U][M][2]
+void orc_multi_detect( void* h, const float* rx, int64_t M, double* est ) { multiDetect( *static_cast<Multi*>( h ), rx, M, est ); }
+
+// theta [U][L]; coeffs [U][cap] of the dictionary rows [lo, used) in stream order.
+int64_t orc_multi_state( void* h, double* theta, double* coeffs, int64_t cap, int64_t* lo, double* var )
+{
+    const Multi* m = static_cast<Multi*>( h );
+    std::copy( m->theta.begin(), m->theta.end(), theta );
+    const int64_t l = m->dictLo( m->used );
+    *lo = l;
+    *var = m->var;
+    const int64_t n = std::min<int64_t>( cap, m->used - l );
+    for ( int u = 0; u < m->U; u++ )
+        for ( int64_t i = 0; i < n; i++ )
+            coeffs[ (int64_t)u * cap + i ] = m->c[ ( l + i ) * m->U + u ];
+    return m->used;
 }
 
 // ---- FP32 restatement (CPU baseline, naive reference schedule) ----

@@ -108,6 +108,13 @@ def lib():
         L.orc_bit_errors.argtypes = [f32p, f32p, i64, ctypes.c_int]
         L.orc_evm.restype = ctypes.c_double
         L.orc_evm.argtypes = [f32p, f32p, i64]
+        L.orc_multi_create.restype = ctypes.c_void_p
+        L.orc_multi_create.argtypes = [P(OrcParams), ctypes.c_int, ctypes.c_int]
+        L.orc_multi_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_multi_train.argtypes = [ctypes.c_void_p, f32p, f32p, i64]
+        L.orc_multi_detect.argtypes = [ctypes.c_void_p, f32p, i64, f64p]
+        L.orc_multi_state.restype = i64
+        L.orc_multi_state.argtypes = [ctypes.c_void_p, f64p, f64p, i64, P(i64), f64p]
         L.orc_omp_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -205,6 +212,50 @@ class Apsm:
         out = np.zeros(u.shape[0])
         self.L.orc_apsm_eval_rows(self.h, pu, u.shape[0], out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         return out
+
+
+class ApsmMulti:
+    """FP64 streamed form for U detectors sharing one pilot stream (the window Gram matrix is
+    kept between steps; kernel values are shared by the users). Each user's result equals
+    Apsm(cfg, N).train(rx, tx[u], incremental=True). tx: complex [U][T]; detect returns
+    float64 [U][M][2]."""
+
+    def __init__(self, cfg, num_antennas, num_users):
+        self.L = lib()
+        self.N, self.U = num_antennas, num_users
+        self.params = make_params(cfg)
+        self.h = self.L.orc_multi_create(ctypes.byref(self.params), num_antennas, num_users)
+        if not self.h:
+            raise ValueError(self.L.orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_multi_destroy(self.h)
+            self.h = None
+
+    def train(self, rx, tx):
+        rx, prx = _f32(as_rx(rx))
+        tx, ptx = _f32(as_tx(np.atleast_2d(tx)))
+        assert rx.shape[0] == self.N and tx.shape[0] == self.U and rx.shape[1] == tx.shape[1]
+        if self.L.orc_multi_train(self.h, prx, ptx, rx.shape[1]) != 0:
+            raise ValueError(self.L.orc_last_error().decode())
+
+    def detect(self, rx):
+        rx, prx = _f32(as_rx(rx))
+        est = np.zeros((self.U, rx.shape[1], 2), np.float64)
+        self.L.orc_multi_detect(self.h, prx, rx.shape[1], est.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return est
+
+    def state(self, cap=1 << 16):
+        theta = np.zeros((self.U, 2 * self.N))
+        coeffs = np.zeros((self.U, cap))
+        lo = ctypes.c_int64()
+        var = ctypes.c_double()
+        used = self.L.orc_multi_state(self.h, theta.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      coeffs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), cap,
+                                      ctypes.byref(lo), ctypes.byref(var))
+        n = min(cap, used - lo.value)
+        return dict(used=used, lo=lo.value, theta=theta, coeffs=coeffs[:, :n].copy(), var=var.value)
 
 
 def lls_train(rx, tx, method="cholesky"):

@@ -78,3 +78,29 @@ def test_batch_rejects_unsupported_instances(require_gpu):
     d = np.zeros((4, 10), np.complex64)
     with pytest.raises(ApsmError):
         b.plan([d], [np.zeros(10, np.complex64)], [d])
+
+
+def test_replan_right_after_a_run(require_gpu):
+    """A plan issued while the previous run is still on the device (no synchronize between
+    them), with other pilot and sample counts, gives the single-handle results of the new
+    inputs."""
+    from libapsm_amd import ApsmBatch, ApsmNomaDetectorFactory, datagen
+    inst = make_batch(6, seed=21)
+    b = ApsmBatch([x["cfg"] for x in inst], [x["N"] for x in inst], [x["U"] for x in inst])
+    b.plan([x["d"]["rx_train"] for x in inst], [x["d"]["tx_train"][:x["U"]] for x in inst], [x["d"]["rx_data"] for x in inst])
+    b.run()
+    b.run()
+    rng = np.random.default_rng(22)
+    new = []
+    for i, x in enumerate(inst):
+        T = int(rng.integers(50, 450))
+        M = int(rng.integers(100, 3000))
+        new.append(datagen.generate(x["N"], max(x["U"], 3), T, M, modulation="QPSK", snr_db=15.0, seed=2000 + i))
+    b.plan([d["rx_train"] for d in new], [d["tx_train"][:x["U"]] for d, x in zip(new, inst)], [d["rx_data"] for d in new])
+    b.synchronize()
+    for i, (x, d) in enumerate(zip(inst, new)):
+        det = ApsmNomaDetectorFactory.build(dict(x["cfg"], otype="APSM"), x["N"], x["U"])
+        det.train(d["rx_train"], d["tx_train"][:x["U"]])
+        ref = det.detect(d["rx_data"])
+        ref = ref[None, :] if x["U"] == 1 else ref
+        np.testing.assert_array_equal(b.estimates(i), ref.astype(np.complex64), err_msg=f"instance {i}")

@@ -203,3 +203,45 @@ def test_pinned_frames_give_the_pageable_results(require_gpu):
     for k, (x, y) in enumerate(zip(a, b)):
         np.testing.assert_array_equal(y.est[0], x.est[0], err_msg=f"frame {k}: pinned vs pageable")
     assert any(np.any(y.est != 0) for y in b)
+
+
+def test_bad_late_frame_fails_before_any_work(require_gpu):
+    """A replay whose last frame cannot be served fails before anything is enqueued; the server
+    stays idle and serves the good frames afterwards with the results of a fresh server."""
+    from libapsm_amd import ApsmError, Frame
+    fr, links = trace(3, 12, rate_hz=20000.0, seed=17)
+    srv = make_server(links, cfg_of, max_pilots=512)
+    good = frames_of(fr)
+    f0 = fr[0]
+    bad = Frame(f0["link"], f0["rx_data"], np.zeros((f0["N"], 600), np.complex64), np.zeros((1, 600), np.complex64),
+                arrival_s=fr[-1]["arrival_s"] + 1e-3)
+    with pytest.raises(ApsmError):
+        srv.replay(frames_of(fr) + [bad])
+    assert srv.stats()["rounds"] == 0  # nothing was launched
+    res = srv.replay(good)
+    assert all(r["status"] == 0 for r in res)
+    ref = frames_of(fr)
+    make_server(links, cfg_of, max_pilots=512).replay(ref)
+    for k, (x, y) in enumerate(zip(ref, good)):
+        np.testing.assert_array_equal(y.est[0], x.est[0], err_msg=f"frame {k}")
+
+
+def test_poll_without_wait_gives_the_replay_results(require_gpu):
+    """Frames in pageable buffers reported by non-waiting polls hold their final estimates."""
+    import time
+    fr, links = trace(4, 20, rate_hz=10000.0, seed=18)
+    a = frames_of(fr)
+    make_server(links, cfg_of).replay(a)
+    srv = make_server(links, cfg_of)
+    b = frames_of(fr)
+    for f in b:
+        srv.submit(f)
+    done, seen = [], {}
+    t_end = time.time() + 60.0
+    while len(done) < len(b) and time.time() < t_end:
+        for f in srv.poll(wait=False):
+            seen[id(f)] = f.est[0].copy()  # read right after the poll reported it
+            done.append(f)
+    assert len(done) == len(b)
+    for k, (x, y) in enumerate(zip(a, b)):
+        np.testing.assert_array_equal(seen[id(y)], x.est[0], err_msg=f"frame {k}")

@@ -284,3 +284,141 @@ def test_norm_ratio_single_entry_closed_form(oracle):
     o.set_state(rows, c, theta, var)
     q = c[0] ** 2 + c[1] ** 2 + 2 * c[0] * c[1] * np.exp(-np.sum(np.abs(x) ** 2) / var)
     assert o.norm_ratio() == pytest.approx((1 - wG) * theta @ theta / (wG * q), rel=1e-12)
+
+
+def _embed(x):
+    """complex [N] -> real rows a = [Re; Im], b = [Im; -Re] (float64)"""
+    x = np.asarray(x, np.complex128)
+    return np.concatenate([x.real, x.imag]), np.concatenate([x.imag, -x.real])
+
+
+def _soft(r, eB):
+    return r - eB if r > eB else (r + eB if r < -eB else 0.0)
+
+
+def _two_complex_pilots(seed):
+    rng = np.random.default_rng(seed)
+    N = 3
+    x = (rng.standard_normal((N, 3)) + 1j * rng.standard_normal((N, 3))).astype(np.complex64)
+    s = np.array([0.7 - 0.2j, -0.4 + 0.9j, 0.3 + 0.5j], np.complex64)
+    return N, x, s
+
+
+def test_window_ramp_two_steps_closed_form(oracle):
+    # Two steps of sampleStep 2 with windowSize >= 2: the window ramps W = 2 -> 4 real rows.
+    # Step 1 (rows 0, 1; W = 2), zero state: f = 0, beta1_t = soft(y_t)/2/(wL|x_t|^2 + wG).
+    # Step 2 (rows 0..3; W = 4), state after step 1 (c_0, c_1 = beta1; c_2 = c_3 = 0):
+    #   f_t = wL * (beta1_0 x_0 + beta1_1 x_1).x_t + wG * (beta1_0 g(0,t) + beta1_1 g(1,t)),
+    #   beta2_t = soft(y_t - f_t)/4/(wL|x_t|^2 + wG), c_t = beta1_t + beta2_t (t < 2) or beta2_t,
+    #   theta = sum_t (beta1_t + beta2_t) x_t.
+    N, x, s = _two_complex_pilots(11)
+    wG, eB, v = 0.45, 0.02, 0.4
+    wL = 1 - wG
+    cfg = dict(windowSize=3, dictionarySize=8, eB=eB, gaussianKernelVariance=v, gaussianKernelWeight=wG)
+    o = oracle.Apsm(cfg, N)
+    o.train(x[:, :2], s[:2])
+    st = o.state()
+    var = st["var"]
+    a0, b0 = _embed(x[:, 0])
+    a1, b1 = _embed(x[:, 1])
+    X = [a0, b0, a1, b1]
+    y = [float(s[0].real), float(s[0].imag), float(s[1].real), float(s[1].imag)]
+    g = lambda i, j: np.exp(-0.5 / var * np.sum((X[i] - X[j]) ** 2))  # noqa: E731
+    den = [wL * X[t] @ X[t] + wG for t in range(4)]
+
+    def two_steps(W1, W2):
+        beta1 = [_soft(y[t], eB) / W1 / den[t] for t in range(2)]
+        th1 = beta1[0] * X[0] + beta1[1] * X[1]
+        f = [wL * th1 @ X[t] + wG * (beta1[0] * g(0, t) + beta1[1] * g(1, t)) for t in range(4)]
+        beta2 = [_soft(y[t] - f[t], eB) / W2 / den[t] for t in range(4)]
+        c = [beta1[0] + beta2[0], beta1[1] + beta2[1], beta2[2], beta2[3]]
+        return np.array(c), sum(c[t] * X[t] for t in range(4))
+
+    c, th = two_steps(2, 4)
+    np.testing.assert_allclose(st["coeffs"], c, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(st["theta"], th, rtol=1e-12, atol=1e-15)
+    # the pin tells a wrong ramp apart: a constant W (2 or 4) or a window taken as the
+    # whole stream in step 1 gives other coefficients
+    for W1, W2 in ((2, 2), (4, 4), (1, 4)):
+        cw, _ = two_steps(W1, W2)
+        assert np.abs(cw - c).max() > 1e-3 * np.abs(c).max()
+    # detection with the final state, written out: f(u) = wL theta.u + wG sum_j c_j g(d_j, u)
+    u_a, u_b = _embed(x[:, 2])
+    fa = wL * th @ u_a + wG * sum(c[j] * np.exp(-0.5 / var * np.sum((X[j] - u_a) ** 2)) for j in range(4))
+    fb = wL * th @ u_b + wG * sum(c[j] * np.exp(-0.5 / var * np.sum((X[j] - u_b) ** 2)) for j in range(4))
+    np.testing.assert_allclose(o.detect(x[:, 2:3])[0], [fa, fb], rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("incremental", [False, True])
+def test_full_window_start_closed_form(oracle, incremental):
+    # startWithFullWindow, windowSize 2 (2w = 4 real rows), sampleStep 2, three pilots:
+    # step 1 at s = 4 projects rows 0..3 from the zero state with W = 4; step 2 at s = 6
+    # projects rows 2..5 (the newest 4) with W = 4 against the state after step 1.
+    N, x, s = _two_complex_pilots(12)
+    wG, eB, v = 0.5, 0.01, 0.3
+    wL = 1 - wG
+    cfg = dict(windowSize=2, dictionarySize=8, eB=eB, gaussianKernelVariance=v, gaussianKernelWeight=wG,
+               startWithFullWindow=True)
+    o = oracle.Apsm(cfg, N)
+    o.train(x, s, incremental=incremental)
+    st = o.state()
+    var = st["var"]
+    X = [r for k in range(3) for r in _embed(x[:, k])]
+    y = [v_ for k in range(3) for v_ in (float(s[k].real), float(s[k].imag))]
+    g = lambda i, j: np.exp(-0.5 / var * np.sum((X[i] - X[j]) ** 2))  # noqa: E731
+    den = [wL * X[t] @ X[t] + wG for t in range(6)]
+
+    def two_steps(W1, W2, rows2):
+        beta1 = [_soft(y[t], eB) / W1 / den[t] for t in range(4)]
+        th1 = sum(beta1[t] * X[t] for t in range(4))
+        c = beta1 + [0.0, 0.0]
+        f = {t: wL * th1 @ X[t] + wG * sum(beta1[j] * g(j, t) for j in range(4)) for t in rows2}
+        for t in rows2:
+            c[t] += _soft(y[t] - f[t], eB) / W2 / den[t]
+        return np.array(c), sum(c[t] * X[t] for t in range(6))
+
+    c, th = two_steps(4, 4, range(2, 6))
+    np.testing.assert_allclose(st["coeffs"], c, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(st["theta"], th, rtol=1e-12, atol=1e-15)
+    for W1, W2, rows2 in ((2, 4, range(2, 6)), (4, 6, range(0, 6)), (4, 2, range(4, 6))):
+        cw, _ = two_steps(W1, W2, rows2)
+        assert np.abs(cw - c).max() > 1e-3 * np.abs(c).max()
+
+
+MULTI_CFGS = [
+    dict(windowSize=10, dictionarySize=200, eB=0.01, gaussianKernelVariance=0.1),
+    dict(windowSize=10, dictionarySize=40, eB=0.01, gaussianKernelVariance=0.1),  # FIFO wrap
+    dict(windowSize=6, dictionarySize=30, eB=0.01, gaussianKernelVariance=0.3, sampleStep=4),
+    dict(windowSize=10, dictionarySize=200, eB=0.01, gaussianKernelVariance=0.1, startWithFullWindow=True),
+    dict(windowSize=8, dictionarySize=100, eB=0.001, gaussianKernelVariance=0.0),  # variance heuristic
+]
+
+
+@pytest.mark.parametrize("cfg", MULTI_CFGS)
+def test_streamed_multi_user_form_equals_single_user_forms(oracle, cfg):
+    # The streamed form (shared stream, window Gram kept between steps) against the
+    # incremental and naive single-user forms, user by user, over two train calls.
+    d = small_problem(N=4, U=3, T=160, M=60, seed=9)
+    m = oracle.ApsmMulti(cfg, 4, 3)
+    m.train(d["rx_train"][:, :90], d["tx_train"][:, :90])
+    m.train(d["rx_train"][:, 90:], d["tx_train"][:, 90:])
+    em, sm = m.detect(d["rx_data"]), m.state()
+    for u in range(3):
+        for inc in (True, False):
+            o = oracle.Apsm(cfg, 4)
+            o.train(d["rx_train"][:, :90], d["tx_train"][u, :90], incremental=inc)
+            o.train(d["rx_train"][:, 90:], d["tx_train"][u, 90:], incremental=inc)
+            so = o.state()
+            assert so["used"] == sm["used"] and so["lo"] == sm["lo"] and so["var"] == sm["var"]
+            tol = 0.0 if inc else 1e-12
+            np.testing.assert_allclose(sm["coeffs"][u], so["coeffs"], rtol=0, atol=tol)
+            np.testing.assert_allclose(sm["theta"][u], so["theta"], rtol=0, atol=tol)
+            np.testing.assert_allclose(em[u], o.detect(d["rx_data"]), rtol=0, atol=tol)
+
+
+def test_streamed_form_rejects_what_it_does_not_restate(oracle):
+    for extra in (dict(trainPasses=2), dict(normConstraint=0.3), dict(llsInitialization=True), dict(trainingShuffle=True)):
+        m = oracle.ApsmMulti(dict(windowSize=4, dictionarySize=50, gaussianKernelVariance=0.1, **extra), 4, 2)
+        d = small_problem(N=4, U=2, T=50, M=4, seed=1)
+        with pytest.raises(ValueError):
+            m.train(d["rx_train"], d["tx_train"][:2])
