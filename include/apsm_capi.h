@@ -120,6 +120,22 @@ int apsm_detect_dev(apsm_handle* h, const float* d_rx, uint32_t M, float* d_est)
  * (cpp/lib/cuda/cuda_eventtimer.cuh:73-133); covers the *_dev calls as well. */
 int apsm_kernel_times(apsm_handle* h, float* train_ms, float* detect_ms);
 
+/* Per-kernel device time of the handle's launches (no counterpart in the reference, whose
+ * timer covers the whole train loop or detect kernel). While enabled, pairs of HIP events on
+ * the handle's stream bracket every launch of the train sequencer (APSM_PROFILE_SEQUENCER,
+ * apsm_train_seq2, one per train block), every block precompute (APSM_PROFILE_BLOCK_PRE: the
+ * entry/Gram/eviction launch and the entry combine of a block) and every detect
+ * (APSM_PROFILE_DETECT) of the blocked train path and the detect entry points.
+ * apsm_profile_read waits for the recorded events and returns, per kind, the summed device
+ * time (ms[APSM_PROFILE_KINDS]) and the number of bracketed launches (launches[...]) since the
+ * previous read, then clears them. */
+#define APSM_PROFILE_SEQUENCER 0
+#define APSM_PROFILE_BLOCK_PRE 1
+#define APSM_PROFILE_DETECT 2
+#define APSM_PROFILE_KINDS 3
+int apsm_profile_enable(apsm_handle* h, int enable);
+int apsm_profile_read(apsm_handle* h, double* ms, uint64_t* launches);
+
 /* Waits for all work enqueued on the handle's stream. */
 int apsm_synchronize(apsm_handle* h);
 

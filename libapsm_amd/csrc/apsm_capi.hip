@@ -265,9 +265,34 @@ struct apsm_handle
     DevBuf<float> rx, tx, est; // staging for the host entry points
     DevBuf<int32_t> perm;
     lls_handle* lls = nullptr;
+    // per-kernel profile (apsm_profile_*): event pairs bracketing launches of a kind
+    bool prof = false;
+    std::vector<hipEvent_t> profEv; // pool, reused after every read
+    size_t profUsed = 0;
+    std::vector<std::array<int, 3>> profMarks; // (kind, start event, end event)
 };
 
 namespace {
+
+// Next event of the handle's profile pool, recorded on the handle's stream (index returned).
+int profRecord( apsm_handle* h )
+{
+    if ( h->profUsed == h->profEv.size() )
+    {
+        hipEvent_t e = nullptr;
+        HIP_OR_THROW( hipEventCreate( &e ) );
+        h->profEv.push_back( e );
+    }
+    const int i = (int)h->profUsed++;
+    HIP_OR_THROW( hipEventRecord( h->profEv[ i ], h->stream ) );
+    return i;
+}
+
+void profMark( apsm_handle* h, int kind, int start, int end )
+{
+    if ( h->prof )
+        h->profMarks.push_back( { kind, start, end } );
+}
 
 void resetDeviceState( apsm_handle* h )
 {
@@ -473,6 +498,7 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
 
         // one launch for the entry share of the first 16 users, the Gram and the eviction
         // shares; further user groups get entry launches of their own; then the combines
+        const int evPre = h->prof ? profRecord( h ) : -1;
         for ( int u0 = 0; u0 < U; u0 += 16 )
         {
             apsm::EntryArgs eu = ea;
@@ -526,6 +552,8 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
         s2.wG = wG;
         s2.eB = h->p.eB;
         const size_t lds = apsm::seq2LdsBytes( newPer, w2, tp );
+        const int evSeq = h->prof ? profRecord( h ) : -1;
+        profMark( h, APSM_PROFILE_BLOCK_PRE, evPre, evSeq );
         const int64_t aR = a0 - rowBase; // local real row of the first active row
         const float* Za = b.Z.ptr + ( aR / 2 ) * h->L;
         const float* Zna = b.Zn.ptr + aR / 2;
@@ -535,6 +563,8 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
             launchSeq2Tp<1>( h, tp, Za, Zna, Ya, Ca, h->Gact.ptr, h->EvF.ptr, s2, lds );
         else
             launchSeq2Tp<2>( h, tp, Za, Zna, Ya, Ca, h->Gact.ptr, h->EvF.ptr, s2, lds );
+        if ( h->prof )
+            profMark( h, APSM_PROFILE_SEQUENCER, evSeq, profRecord( h ) );
         h->a0Next = a0Next;
     }
 }
@@ -792,6 +822,7 @@ void detectDev( apsm_handle* h, const float* d_rx, int64_t M, float* d_est )
     const float* Zn = G > 0 ? b.Zn.ptr + off : nullptr;
     const float* C = G > 0 ? b.C.ptr + 2 * off : nullptr;
     HIP_OR_THROW( hipEventRecord( h->ev2, h->stream ) );
+    const int evDet = h->prof ? profRecord( h ) : -1;
     for ( int u0 = 0; u0 < h->U; u0 += 16 )
     {
         const int Ug = std::min( 16, h->U - u0 );
@@ -817,6 +848,8 @@ void detectDev( apsm_handle* h, const float* d_rx, int64_t M, float* d_est )
         else
             launchDetectU<64>( h, d_rx, M, est, Z, Zn, Cg, ld, th, G, Ug );
     }
+    if ( h->prof )
+        profMark( h, APSM_PROFILE_DETECT, evDet, profRecord( h ) );
     HIP_OR_THROW( hipEventRecord( h->ev3, h->stream ) );
     h->detectTimed = true;
 }
@@ -985,6 +1018,8 @@ void apsm_destroy( apsm_handle* h )
     h->perm.release();
     if ( h->lls )
         lls_destroy( h->lls );
+    for ( hipEvent_t e : h->profEv )
+        (void)hipEventDestroy( e );
     if ( h->ev0 )
         (void)hipEventDestroy( h->ev0 );
     if ( h->ev1 )
@@ -1006,6 +1041,37 @@ int apsm_kernel_times( apsm_handle* h, float* train_ms, float* detect_ms )
             *train_ms = h->trainTimed ? elapsedMs( h->ev0, h->ev1 ) : 0.f;
         if ( detect_ms )
             *detect_ms = h->detectTimed ? elapsedMs( h->ev2, h->ev3 ) : 0.f;
+    } );
+}
+
+int apsm_profile_enable( apsm_handle* h, int enable )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h != nullptr, "null handle" );
+        h->prof = enable != 0;
+    } );
+}
+
+int apsm_profile_read( apsm_handle* h, double* ms, uint64_t* launches )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( h && ms && launches, "null argument" );
+        for ( int k = 0; k < APSM_PROFILE_KINDS; k++ )
+        {
+            ms[ k ] = 0.0;
+            launches[ k ] = 0;
+        }
+        if ( h->profUsed > 0 )
+            HIP_OR_THROW( hipEventSynchronize( h->profEv[ h->profUsed - 1 ] ) );
+        for ( const auto& m : h->profMarks )
+        {
+            float t = 0.f;
+            HIP_OR_THROW( hipEventElapsedTime( &t, h->profEv[ m[ 1 ] ], h->profEv[ m[ 2 ] ] ) );
+            ms[ m[ 0 ] ] += t;
+            launches[ m[ 0 ] ]++;
+        }
+        h->profMarks.clear();
+        h->profUsed = 0;
     } );
 }
 

@@ -103,6 +103,21 @@ def kernel_variance_for_sigma(sigma, num_antennas):
     return sigma * sigma / float(num_antennas * num_antennas)
 
 
+def median_pair_variance(rx, pairs=20000, seed=0):
+    """sigma^2 = half the median squared distance |x_i - x_j|^2 over random pairs of the
+    complex samples rx [N][T] (so that exp(-d^2 / 2 sigma^2) = 1/e at the median distance).
+    The reference's own heuristic (cpp/lib/apsm/apsm_noma_detector.cu:194-218: the 70th
+    percentile of sum_n |x_i,n - x_j,n| times T^(-1/5), squared, not scaled with N) gives a
+    kernel far narrower than the pilot spacing at many antennas, where the Gaussian term then
+    vanishes from f."""
+    rx = np.asarray(rx)
+    T = rx.shape[1]
+    idx = np.random.Generator(np.random.PCG64(seed)).integers(0, T, size=(pairs, 2))
+    idx = idx[idx[:, 0] != idx[:, 1]]
+    d2 = np.sum(np.abs(rx[:, idx[:, 0]].astype(np.complex128) - rx[:, idx[:, 1]]) ** 2, axis=0)
+    return float(np.median(d2) / 2.0)
+
+
 # Workloads (sizes from the benchmark configurations).
 WORKLOADS = {
     # 64 antennas, 12 QPSK users at 12 dB; dictionary of 16384 centres; 8M detect samples

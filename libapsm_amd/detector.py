@@ -235,6 +235,19 @@ class ApsmNomaDetector(NomaDetector):
         check(lib().apsm_kernel_times(self._h, ctypes.byref(t), ctypes.byref(d)))
         return t.value, d.value
 
+    PROFILE_KINDS = ("sequencer", "block_pre", "detect")
+
+    def profile_enable(self, enable=True):
+        """Bracket every sequencer, block-precompute and detect launch with HIP events."""
+        check(lib().apsm_profile_enable(self._h, int(bool(enable))))
+
+    def profile_read(self):
+        """{kind: (device ms summed, launches)} since the previous read (waits for the events)."""
+        ms = (ctypes.c_double * 3)()
+        n = (ctypes.c_uint64 * 3)()
+        check(lib().apsm_profile_read(self._h, ms, n))
+        return {k: (ms[i], n[i]) for i, k in enumerate(self.PROFILE_KINDS)}
+
     def stream(self):
         return lib().apsm_stream(self._h)
 

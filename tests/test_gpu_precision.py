@@ -100,3 +100,51 @@ def test_switching_precision_keeps_the_trained_state(require_gpu):
         np.testing.assert_array_equal(det.detect(d["rx_data"]), run_mode(cfg, 16, 1, d, mode)[0], err_msg=mode)
     with pytest.raises(ValueError):
         det.setDetectPrecision("int8")
+
+
+# Bounds of the relative soft-output RMS error against the FP64 oracle where the Gaussian term
+# carries at least GSHARE_MIN of f: 16-bit cross-term inputs perturb each squared distance by
+# about 2^-8 (bf16) or 2^-11 (fp16) of |x||z|, i.e. each kernel value by that much times
+# |x||z| / (2 sigma^2); FP32 stays at rounding level. The 16-bit error must also stand clearly
+# above the FP32 error (otherwise the test could not see a wrong rounding).
+ACTIVE_BOUNDS = {"fp32": 1e-4, "bf16": 1e-2, "fp16": 2e-3}
+GSHARE_MIN = 0.10
+
+
+@pytest.mark.parametrize("N", [32, 64])
+def test_half_precision_where_the_gaussian_term_acts(require_gpu, oracle, N):
+    """12 QPSK users at 12 dB (the mixed-precision workload's signal), gaussianKernelWeight
+    0.99 and sigma^2 at half the median squared pilot-pair distance, so that the Gaussian term
+    is a large share of f and the 16-bit cross terms show in the soft outputs."""
+    from libapsm_amd import ApsmNomaDetectorFactory, datagen
+    d = datagen.generate(N, 12, 2048, 8192, modulation="QPSK", snr_db=12.0, seed=40 + N)
+    sigma2 = datagen.median_pair_variance(d["rx_train"])
+    cfg = dict(windowSize=80, dictionarySize=2048, eB=0.01, gaussianKernelVariance=sigma2 / (N * N),
+               gaussianKernelWeight=0.99)
+    print(f"N={N} sigma2 {sigma2:.2f}", flush=True)
+    o = oracle.Apsm(cfg, N)
+    o.train(d["rx_train"], d["tx_train"][0], incremental=True)
+    est_o = o.detect(d["rx_data"])
+    th = o.state()["theta"]
+    x = d["rx_data"].astype(np.complex128)
+    lin = 0.01 * np.stack([th[:N] @ x.real + th[N:] @ x.imag, th[:N] @ x.imag - th[N:] @ x.real], -1)
+    rms_o = np.sqrt(np.mean(est_o ** 2))
+    gshare = np.sqrt(np.mean((est_o - lin) ** 2)) / rms_o
+    print(f"Gaussian share of f: {gshare:.3f}", flush=True)
+    assert gshare >= GSHARE_MIN
+    det = ApsmNomaDetectorFactory.build(dict(cfg, otype="APSM"), N)
+    det.train(d["rx_train"], d["tx_train"][0])
+    tx = to_ri(d["tx_data"][0])
+    e_o = oracle.symbol_errors(est_o, tx, 2)
+    rel = {}
+    for mode in ("fp32", "bf16", "fp16"):
+        det.setDetectPrecision(mode)
+        g = to_ri(det.detect(d["rx_data"])).astype(np.float64)
+        rel[mode] = np.sqrt(np.mean((g - est_o) ** 2)) / rms_o
+        e = oracle.symbol_errors(g, tx, 2)
+        print(f"N={N} {mode}: rel soft RMS err {rel[mode]:.3e}, max {np.abs(g - est_o).max():.3e}; "
+              f"symbol errors {e} (oracle {e_o})", flush=True)
+        assert rel[mode] <= ACTIVE_BOUNDS[mode], mode
+        assert abs(e - e_o) <= 0.01 * e_o + 3, mode
+    assert rel["bf16"] > 5 * rel["fp32"]
+    assert rel["bf16"] > rel["fp16"]
