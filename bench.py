@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--batched-instances", type=int, default=512)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision large-instance measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--no-small", action="store_true", help="skip the small plain workload (GPU, oracle SER, host cores)")
     ap.add_argument("--serving-frames", type=int, default=600)
     ap.add_argument("--serving-links", type=int, default=2048)
     ap.add_argument("--only", choices=["batched", "mixed", "serving", "copy"], default=None,
@@ -89,50 +90,124 @@ def naive_evals(W2, H, R2, q=2):
     return float(np.sum(np.minimum(W2, s) * np.minimum(s, H)))
 
 
-def cpu_baseline(w, data, budget_s):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(w, data, budget_s, threads=None):
+    """The reference algorithm (FP32 restatement, the reference's schedule: every step
+    evaluates every window row against the whole dictionary) timed on host cores over a
+    bounded sample: one user trained on the first Ts pilots and Ms samples detected, Ts and Ms
+    doubled until a part takes a quarter of the budget (or covers the whole workload); the
+    per-kernel-evaluation times are scaled to the whole workload (all users)."""
     from libapsm_amd import datagen
     from oracle import pyoracle
 
     cfg = {k: v for k, v in datagen.workload_config(w).items() if k != "otype"}
-    N = w["N"]
+    N, T, M = w["N"], w["T"], w["M"]
     W2, H = 2 * w["window"], 2 * w["dict_size"]
-    threads = pyoracle.threads()
-    # calibrate: a short naive train on the first pilots, then detect a few samples
-    Ts = 256
-    while True:
-        o = pyoracle.Apsm(cfg, N, fp32=True)
-        t0 = time.perf_counter()
-        o.train(data["rx_train"][:, :Ts], data["tx_train"][0, :Ts])
-        t_train = time.perf_counter() - t0
-        if t_train > budget_s / 4 or Ts >= 8192:
-            break
-        Ts *= 2
-    ev_train = naive_evals(W2, H, 2 * Ts)
-    Ms = 512
-    while True:
-        t0 = time.perf_counter()
-        o.detect(data["rx_data"][:, :Ms])
-        t_det = time.perf_counter() - t0
-        if t_det > budget_s / 4 or Ms >= 65536:
-            break
-        Ms *= 2
-    Gs = min(2 * Ts, H)
-    ev_det = 2.0 * Ms * Gs
-    per_train_eval = t_train / ev_train
-    per_det_eval = t_det / ev_det
+    user = w["users"][0]
+    prev = pyoracle.threads()
+    if threads:
+        pyoracle.set_threads(threads)
+    try:
+        used_threads = pyoracle.threads()
+        Ts = min(256, T)
+        while True:
+            o = pyoracle.Apsm(cfg, N, fp32=True)
+            t0 = time.perf_counter()
+            o.train(data["rx_train"][:, :Ts], data["tx_train"][user, :Ts])
+            t_train = time.perf_counter() - t0
+            if t_train > budget_s / 4 or Ts >= min(8192, T):
+                break
+            Ts = min(2 * Ts, T)
+        Ms = min(512, M)
+        while True:
+            t0 = time.perf_counter()
+            o.detect(data["rx_data"][:, :Ms])
+            t_det = time.perf_counter() - t0
+            if t_det > budget_s / 4 or Ms >= min(65536, M):
+                break
+            Ms = min(2 * Ms, M)
+    finally:
+        pyoracle.set_threads(prev)
+    per_train_eval = t_train / naive_evals(W2, H, 2 * Ts)
+    per_det_eval = t_det / (2.0 * Ms * min(2 * Ts, H))
     U = len(w["users"])
-    full_train = naive_evals(W2, H, 2 * w["T"]) * per_train_eval
-    full_det = 2.0 * w["M"] * min(2 * w["T"], H) * per_det_eval
+    full_train = naive_evals(W2, H, 2 * T) * per_train_eval
+    full_det = 2.0 * M * min(2 * T, H) * per_det_eval
     total = U * (full_train + full_det)
     return {
-        "value": w["M"] * U / total,
+        "value": M * U / total,
         "unit": "detected symbols/s",
-        "cores": threads,
+        "cores": used_threads,
         "kind": "port",
-        "sample": (f"FP32 restatement, reference schedule; user 0 trained on the first {Ts} pilots "
-                   f"({t_train:.2f} s) and {Ms} samples detected ({t_det:.2f} s); per-kernel-evaluation "
-                   f"times scaled to {U} users x ({w['T']} pilots, {w['M']} samples): "
-                   f"train {U * full_train:.0f} s + detect {U * full_det:.0f} s"),
+        "cpu_model": cpu_model(),
+        "measured_s": t_train + t_det,
+        "extrapolated_s": total,
+        "sample": (f"FP32 restatement, reference schedule, {used_threads} thread(s); user {user} trained on the first {Ts} "
+                   f"of {T} pilots ({t_train:.2f} s) and {Ms} of {M} samples detected against that dictionary "
+                   f"({t_det:.2f} s); per-kernel-evaluation times scaled to {U} user(s) x ({T} pilots, {M} samples): "
+                   f"train {U * full_train:.1f} s + detect {U * full_det:.1f} s"),
+    }
+
+
+def bench_small_plain(steps, budget_s):
+    """The small plain workload (8 antennas, 6 superimposed QPSK users at 10 dB, user 1
+    detected, windowSize 80, sigma 0.5, 20k pilots, 80k samples) on the GPU, its symbol errors
+    beside the FP64 oracle's (tests/golden/scale_c1.npz, all 80k samples), and the reference
+    algorithm on the host cores (all cores and one core)."""
+    from libapsm_amd import ApsmNomaDetectorFactory, datagen, lib
+    from libapsm_amd._lib import DeviceArray, check
+
+    w = datagen.WORKLOADS["plain-cpu-8ant"]
+    N, U, T, M = w["N"], len(w["users"]), w["T"], w["M"]
+    data = datagen.generate(N, w["U_tx"], T, M, modulation=w["modulation"], snr_db=w["snr_db"], seed=w["seed"],
+                            large_scale_db=w["large_scale_db"])
+
+    def to_dev(z):
+        return DeviceArray(np.ascontiguousarray(z, dtype=np.complex64).view(np.float32))
+
+    d_rxt, d_txt, d_rxd = to_dev(data["rx_train"]), to_dev(data["tx_train"][w["users"]]), to_dev(data["rx_data"])
+    d_est = DeviceArray.empty((U, M, 2), np.float32)
+    det = ApsmNomaDetectorFactory.build(datagen.workload_config(w), N, U)
+
+    def step():
+        det.resetState()
+        det.train_device(d_rxt.ptr, d_txt.ptr, T)
+        det.detect_device(d_rxd.ptr, M, d_est.ptr)
+
+    step()
+    check(lib().apsm_device_synchronize())
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    check(lib().apsm_device_synchronize())
+    ms_step = 1000.0 * (time.perf_counter() - t0) / steps
+    est = d_est.numpy()
+    errs = sum(datagen.symbol_errors(est[k, :, 0] + 1j * est[k, :, 1], data["tx_data"][u], data["bps"])
+               for k, u in enumerate(w["users"]))
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "scale_c1.npz"), allow_pickle=False)
+    e_orc = int(fx["symbol_errors"].sum())
+    return {
+        "workload": "plain-cpu-8ant",
+        "value": M * U / (ms_step / 1000.0),
+        "unit": "detected symbols/s",
+        "ms_per_step": ms_step,
+        "ser": errs / (M * U),
+        "ser_oracle": e_orc / (M * U),
+        "symbol_errors": int(errs),
+        "symbol_errors_oracle": e_orc,
+        "cpu_baseline": cpu_baseline(w, data, budget_s),
+        "cpu_baseline_one_core": cpu_baseline(w, data, budget_s / 2, threads=1),
     }
 
 
@@ -583,6 +658,13 @@ def main():
         ],
         "ser": ser,
     }
+    fx_path = os.path.join(ROOT, "tests", "golden", "scale_c2.npz")
+    if args.workload == "plain-16ant-16qam" and os.path.exists(fx_path):
+        # the FP64 oracle's symbol errors on the same data (tests/golden/make_scale_fixtures.py)
+        e_orc = int(np.load(fx_path, allow_pickle=False)["symbol_errors"].sum())
+        e_gpu = int(round(ser * est_c.size))
+        out["ser_oracle"] = e_orc / est_c.size
+        out["ser_rel_diff"] = abs(e_gpu - e_orc) / max(1, e_orc)
     if not args.no_lls:
         lls = LlsNomaDetector(N)
         d_txu = to_dev(data["tx_train"][w["users"][0]])
@@ -599,8 +681,11 @@ def main():
                       "apsm_ser_same_user": datagen.symbol_errors(est_c[0], tx_data[0], data["bps"]) / M}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, data, args.cpu_seconds)
-    if not args.no_batched or not args.no_mixed:
+        out["cpu_baseline"]["one_core"] = cpu_baseline(w, data, args.cpu_seconds / 2, threads=1)
+    if not args.no_batched or not args.no_mixed or not args.no_small:
         del d_rxt, d_txt, d_rxd, d_est, det
+    if not args.no_small:
+        out["small_plain"] = bench_small_plain(max(1, args.steps), args.cpu_seconds)
     if not args.no_batched:
         out["batched"] = bench_batched(args.batched_instances, max(1, args.steps))
     if not args.no_mixed:

@@ -5,6 +5,10 @@ form with the window Gram kept between steps) computes on a seeded synthetic wor
 the GPU test regenerates with libapsm_amd.datagen (SHA-256 digests of the generated arrays
 are stored, so a generator change fails the test instead of comparing different data):
 
+  scale_c1.npz  small plain workload (the CPU-runnable one): 8 antennas, 6 superimposed QPSK
+                users at 10 dB, user 1 detected, windowSize 80, 20k pilots (dictionary
+                20000: no eviction), 80k data samples. Symbol errors over all samples, soft
+                outputs on the first 16384, theta.
   scale_c2.npz  plain workload: 16 antennas, 10 superimposed 16QAM users with large-scale
                 gains in [-6, 0] dB at 15 dB SNR, all 10 users detected by one handle,
                 windowSize 160, dictionary 4096 (FIFO over 200k pilots: 200k projection
@@ -20,7 +24,7 @@ are stored, so a generator change fails the test instead of comparing different 
                 and its last frame.
 
 Run from the repository root (about 6 minutes on 8 cores):
-    python tests/golden/make_scale_fixtures.py [c2] [c3] [c5]
+    python tests/golden/make_scale_fixtures.py [c1] [c2] [c3] [c5]
 """
 import hashlib
 import os
@@ -53,6 +57,29 @@ def digest(*arrays):
 def to_c(est):
     """float64 [..][2] -> complex128"""
     return est[..., 0] + 1j * est[..., 1]
+
+
+def c1_data():
+    w = datagen.WORKLOADS["plain-cpu-8ant"]
+    d = datagen.generate(w["N"], w["U_tx"], w["T"], w["M"], modulation=w["modulation"], snr_db=w["snr_db"], seed=w["seed"],
+                         large_scale_db=w["large_scale_db"])
+    return w, d
+
+
+def make_c1():
+    t0 = time.time()
+    w, d = c1_data()
+    cfg = {k: v for k, v in datagen.workload_config(w).items() if k != "otype"}
+    users = w["users"]
+    o = pyoracle.ApsmMulti(cfg, w["N"], len(users))
+    o.train(d["rx_train"], d["tx_train"][users])
+    est = o.detect(d["rx_data"])
+    errs = np.array([datagen.symbol_errors(to_c(est[k]), d["tx_data"][u], d["bps"]) for k, u in enumerate(users)], np.int64)
+    st = o.state()
+    np.savez(os.path.join(HERE, "scale_c1.npz"), digest=np.array(digest(d["rx_train"], d["tx_train"], d["rx_data"])),
+             symbol_errors=errs, samples=np.int64(w["M"]), soft=est[:, :SOFT_SLICE].astype(np.float32), theta=st["theta"],
+             used=np.int64(st["used"]))
+    print(f"c1 done {time.time() - t0:.0f} s: symbol errors {errs.tolist()} of {w['M']}", flush=True)
 
 
 def c2_data():
@@ -131,7 +158,7 @@ def make_c5():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["c2", "c3", "c5"]
+    which = sys.argv[1:] or ["c1", "c2", "c3", "c5"]
     print(f"oracle threads: {pyoracle.threads()}", flush=True)
     for name in which:
-        {"c2": make_c2, "c3": make_c3, "c5": make_c5}[name]()
+        {"c1": make_c1, "c2": make_c2, "c3": make_c3, "c5": make_c5}[name]()

@@ -20,6 +20,16 @@ def load(name):
     return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
 
 
+def test_small_plain_workload_fixture():
+    from libapsm_amd import datagen
+    fx = load("scale_c1.npz")
+    w = datagen.WORKLOADS["plain-cpu-8ant"]
+    d = datagen.generate(w["N"], w["U_tx"], w["T"], w["M"], modulation=w["modulation"], snr_db=w["snr_db"], seed=w["seed"],
+                         large_scale_db=w["large_scale_db"])
+    assert digest(d["rx_train"], d["tx_train"], d["rx_data"]) == str(fx["digest"])
+    assert fx["soft"].shape == (len(w["users"]), 16384, 2) and int(fx["used"]) == 2 * w["T"]
+
+
 def test_plain_workload_fixture():
     from libapsm_amd import datagen
     fx = load("scale_c2.npz")

@@ -1,6 +1,8 @@
 """Parity at full workload sizes: the HIP path against FP64-oracle fixtures
 (tests/golden/make_scale_fixtures.py writes them with the streamed FP64 oracle).
 
+  * small plain workload (8 antennas, 6 QPSK users, user 1 detected, windowSize 80, 20k
+    pilots, 80k data samples);
   * plain workload (16 antennas, 10 superimposed 16QAM users, one 10-user handle,
     windowSize 160, dictionary 4096 as a FIFO over 200k pilots = 200k projection steps,
     2M data samples);
@@ -62,6 +64,28 @@ def check_errors(gpu, orc, what):
         bound = SER_REL * o if o >= 100 else SER_ABS_FLOOR
         assert abs(int(g) - int(o)) <= bound, what
     assert abs(int(gpu.sum()) - int(orc.sum())) <= max(SER_REL * orc.sum(), SER_ABS_FLOOR), what
+
+
+def test_small_plain_workload_matches_oracle(require_gpu):
+    from libapsm_amd import ApsmNomaDetectorFactory, datagen
+    fx = fixture("scale_c1.npz")
+    w = datagen.WORKLOADS["plain-cpu-8ant"]
+    d = datagen.generate(w["N"], w["U_tx"], w["T"], w["M"], modulation=w["modulation"], snr_db=w["snr_db"], seed=w["seed"],
+                         large_scale_db=w["large_scale_db"])
+    assert digest(d["rx_train"], d["tx_train"], d["rx_data"]) == str(fx["digest"]), "generator output changed"
+    users = w["users"]
+    det = ApsmNomaDetectorFactory.build(datagen.workload_config(w), w["N"], len(users))
+    det.train(d["rx_train"], d["tx_train"][users])
+    est = np.atleast_2d(det.detect(d["rx_data"]))
+    st = det.getTrainingState()
+    assert st["used_rows"] == int(fx["used"])
+    dth = np.abs(st["theta"].astype(np.float64) - fx["theta"]).max()
+    print(f"small plain workload: theta max abs diff {dth:.3e}", flush=True)
+    assert dth <= 1e-3
+    for k, u in enumerate(users):
+        check_soft(est[k, :fx["soft"].shape[1]], fx["soft"][k], d["bps"], f"user {u}")
+    errs = [datagen.symbol_errors(est[k], d["tx_data"][u], d["bps"]) for k, u in enumerate(users)]
+    check_errors(errs, fx["symbol_errors"], "small plain workload")
 
 
 def test_plain_workload_matches_oracle(require_gpu):
