@@ -198,6 +198,20 @@ int lls_create_on_stream(uint32_t num_antennas, void* stream, lls_handle** out);
 int lls_train_lowlevel(lls_handle* h, const float* d_A, const float* d_y, int32_t num_input_dims,
                        int32_t num_output_dims, int32_t num_samples);
 
+/* lls_train_lowlevel without any device allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). Same arithmetic and kernels as lls_train_lowlevel (the same
+ * weights for the same inputs), but no handle: the solved weights go to d_w (num_input_dims
+ * floats) and the status to *d_status (0, or -1 when the normal matrix is not positive
+ * definite), both device memory, enqueued on `stream` (a hipStream_t; NULL is the null stream)
+ * with no host synchronisation. Its scratch is a fixed 32 MiB per device, held until the
+ * process ends and shared by every call on that device (calls that overlap on one device
+ * race); a problem whose partial sums need more (about nPart * n * (n+1) * 4 bytes with
+ * nPart = ceil(num_samples / floor(65536 / (4 * (n+1)))) plus n * (n+1) * 8) is rejected with
+ * APSM_ERR_ARG before anything is enqueued: about 3.9M rows at 32 unknowns, 64k rows at 128. */
+int lls_train_lowlevel_noalloc(const float* d_A, const float* d_y, int32_t num_input_dims, int32_t num_output_dims,
+                               int32_t num_samples, float* d_w, int32_t* d_status, void* stream);
+
 /* detectLowLevel (cpp/lib/noma/lls_detector.cu:207-218): est = alpha * A w + beta * est on
  * the handle stream (est is not read when beta == 0). */
 int lls_detect_lowlevel(lls_handle* h, const float* d_A, float* d_est, int32_t num_input_dims,

@@ -88,6 +88,7 @@ SIGNATURES = {
     "apsm_detect_dev": (ctypes.c_int, [_vp, _vp, _u32, _vp]),
     "apsm_kernel_times": (ctypes.c_int, [_vp, _f32p, _f32p]),
     "apsm_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lls_train_lowlevel_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp]),
     "apsm_profile_read": (ctypes.c_int, [_vp, _P(ctypes.c_double), _P(ctypes.c_uint64)]),
     "apsm_synchronize": (ctypes.c_int, [_vp]),
     "apsm_stream": (_vp, [_vp]),

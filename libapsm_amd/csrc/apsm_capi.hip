@@ -20,6 +20,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 #include <stdexcept>
@@ -223,6 +224,57 @@ void llsTrainColMajor( lls_handle* h, const float* d_A, const float* d_y, int n,
                         h->part.ptr );
     HIP_OR_THROW( hipGetLastError() );
     llsReduceSolve( h, nPart, n );
+}
+
+// ---- least squares without allocation (lls_train_lowlevel_noalloc) ----
+// Device scratch of the allocation-free least-squares entry point: the per-chunk FP32 partial
+// normal equations, then their FP64 sum. One copy per device, held until the process ends;
+// every call on a device uses the same copy (calls that overlap on one device race). Host
+// code never passes the array's name to a kernel or copy: its device address comes from
+// hipGetSymbolAddress, once per device (llsScratchOnCurrentDevice).
+constexpr size_t kLlsScratchBytes = size_t( 32 ) << 20;
+static __device__ __align__( 256 ) unsigned char scratch[ kLlsScratchBytes ];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_llsScratchMutex;
+unsigned char* g_llsScratch[ kMaxScratchDevices ] = {};
+
+unsigned char* llsScratchOnCurrentDevice()
+{
+    int dev = 0;
+    HIP_OR_THROW( hipGetDevice( &dev ) );
+    ARG_CHECK( dev >= 0 && dev < kMaxScratchDevices, "device index beyond the scratch table" );
+    std::lock_guard<std::mutex> lock( g_llsScratchMutex );
+    if ( !g_llsScratch[ dev ] )
+    {
+        void* ptr = nullptr;
+        HIP_OR_THROW( hipGetSymbolAddress( &ptr, HIP_SYMBOL( scratch ) ) );
+        ARG_CHECK( ptr != nullptr, "no device address for the least-squares scratch" );
+        g_llsScratch[ dev ] = static_cast<unsigned char*>( ptr );
+    }
+    return g_llsScratch[ dev ];
+}
+
+// Scratch layout of one no-allocation solve of n unknowns over m rows (the chunking of
+// llsTrainColMajor): nPart * E FP32 partials, then E FP64 sums at a 256-byte boundary.
+struct LlsScratchPlan
+{
+    int E = 0, nPart = 0;
+    int64_t chunk = 0;
+    size_t gramOff = 0, bytes = 0;
+};
+
+LlsScratchPlan llsScratchPlan( int n, int64_t m )
+{
+    LlsScratchPlan sp;
+    sp.E = n * ( n + 1 );
+    sp.chunk = std::max<int64_t>( 1, (int64_t)( apsm::kLlsLdsBudget / ( ( n + 1 ) * sizeof( float ) ) ) );
+    const int64_t nPart = ( m + sp.chunk - 1 ) / sp.chunk;
+    const size_t partBytes = (size_t)nPart * sp.E * sizeof( float );
+    sp.gramOff = ( partBytes + 255 ) & ~size_t( 255 );
+    sp.bytes = sp.gramOff + (size_t)sp.E * sizeof( double );
+    sp.nPart = nPart < ( 1 << 30 ) ? (int)nPart : ( 1 << 30 );
+    return sp;
 }
 
 void llsCheckStatus( lls_handle* h )
@@ -1490,6 +1542,41 @@ int lls_train_lowlevel( lls_handle* h, const float* d_A, const float* d_y, int32
         llsTrainColMajor( h, d_A, d_y, num_input_dims, num_samples );
         llsCheckStatus( h );
         h->trained = true;
+    } );
+}
+
+int lls_train_lowlevel_noalloc( const float* d_A, const float* d_y, int32_t num_input_dims, int32_t num_output_dims,
+                                int32_t num_samples, float* d_w, int32_t* d_status, void* stream )
+{
+    return guarded( [ & ] {
+        ARG_CHECK( d_A && d_y && d_w && d_status, "null argument" );
+        ARG_CHECK( num_output_dims == 1, "LLS solves for one output dimension" );
+        ARG_CHECK( num_samples > 0 && num_input_dims > 0, "LLS train needs at least one row and one column" );
+        ARG_CHECK( num_input_dims <= kLlsMaxDims, "LLS supports at most 128 input dimensions" );
+        const int n = num_input_dims;
+        const int64_t m = num_samples;
+        const LlsScratchPlan sp = llsScratchPlan( n, m );
+        if ( sp.bytes > kLlsScratchBytes )
+            throw ApsmError { APSM_ERR_ARG, "least-squares problem needs " + std::to_string( sp.bytes ) +
+                                                " bytes of scratch; the allocation-free entry point holds " +
+                                                std::to_string( kLlsScratchBytes ) };
+        unsigned char* base = llsScratchOnCurrentDevice();
+        float* part = reinterpret_cast<float*>( base );
+        double* gram = reinterpret_cast<double*>( base + sp.gramOff );
+        hipStream_t st = static_cast<hipStream_t>( stream );
+        // every scratch entry a kernel reads was written before it in this call (partials by
+        // the normal-equation kernel, sums by the reduction): nothing needs clearing
+        hipLaunchKernelGGL( apsm::lls_normal_colmajor, dim3( sp.nPart ), dim3( apsm::kLlsThreads ),
+                            (size_t)sp.chunk * ( n + 1 ) * sizeof( float ), st, d_A, d_y, n, m, sp.chunk, part );
+        HIP_OR_THROW( hipGetLastError() );
+        hipLaunchKernelGGL( apsm::lls_reduce_partials, dim3( gridFor( sp.E, 256 ) ), dim3( 256 ), 0, st, part, sp.nPart, sp.E, gram );
+        HIP_OR_THROW( hipGetLastError() );
+        const size_t ldsSolve = (size_t)sp.E * sizeof( double );
+        if ( ldsSolve > 64 * 1024 )
+            HIP_OR_THROW(
+                hipFuncSetAttribute( (const void*)apsm::lls_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsSolve ) );
+        hipLaunchKernelGGL( apsm::lls_solve, dim3( 1 ), dim3( apsm::kSolveThreads ), ldsSolve, st, gram, n, d_w, d_status );
+        HIP_OR_THROW( hipGetLastError() );
     } );
 }
 

@@ -368,6 +368,16 @@ class LlsNomaDetector(NomaDetector):
         check(lib().lls_detect_lowlevel(self._h, ctypes.c_void_p(rxSigMat), ctypes.c_void_p(estMat), int(numInputDims),
                                         int(numOutputDims), int(numSamples), float(alpha), float(beta)))
 
+    @staticmethod
+    def trainLowLevelNoAlloc(A, y, w, status, numInputDims, numOutputDims, numSamples, stream=None):
+        """trainLowLevel without a handle and without device allocation (fixed per-device
+        scratch): device pointers A (column-major numSamples x numInputDims), y (numSamples),
+        outputs w (numInputDims floats) and status (one int32: 0, or -1 if not positive
+        definite); enqueued on `stream` (hipStream_t or None) without synchronisation."""
+        check(lib().lls_train_lowlevel_noalloc(ctypes.c_void_p(A), ctypes.c_void_p(y), int(numInputDims),
+                                               int(numOutputDims), int(numSamples), ctypes.c_void_p(w),
+                                               ctypes.c_void_p(status), ctypes.c_void_p(stream)))
+
     def computeSignalPower(self, data, numValues):
         """sqrt(sum d^2 / n) of a device array (lls_detector.cu:236-242)."""
         out = ctypes.c_float()

@@ -105,3 +105,18 @@ def test_factory_validation_matches_reference():
     assert par.dictionarySize == 685 and par.windowSize == 20 and par.sampleStep == 2
     assert par.detectPrecision == 0
     assert ApsmNomaDetectorFactory.params_from_config({"otype": "APSM", "detectPrecision": "fp16"}).detectPrecision == 4
+
+
+def test_noalloc_least_squares_checks_sizes_before_any_device_call():
+    """The allocation-free least-squares entry point rejects a problem beyond its fixed scratch
+    (and bad arguments) on the host, before touching the device (runs without a GPU)."""
+    import ctypes
+
+    from libapsm_amd._lib import APSM_ERR_ARG, lib
+    L = lib()
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: the call fails before any launch
+    assert L.lls_train_lowlevel_noalloc(fake, fake, 128, 1, 1000000, fake, fake, None) == APSM_ERR_ARG
+    assert b"scratch" in L.apsm_last_error()
+    assert L.lls_train_lowlevel_noalloc(fake, fake, 129, 1, 100, fake, fake, None) == APSM_ERR_ARG
+    assert L.lls_train_lowlevel_noalloc(fake, fake, 32, 2, 100, fake, fake, None) == APSM_ERR_ARG
+    assert L.lls_train_lowlevel_noalloc(None, fake, 32, 1, 100, fake, fake, None) == APSM_ERR_ARG
