@@ -275,6 +275,12 @@ typedef struct apsm_server_config
 {
     uint64_t hbm_budget; /* device bytes for link states and the frame arena together */
     uint64_t frame_arena; /* device bytes of the frame arena (part of hbm_budget) */
+    /* Detect batching (0 selects the default): a batch of newly admitted frames closes at
+     * batch_max_frames frames (default 32) or batch_max_bytes bytes of frame data (default
+     * 64 MiB); at most detects_in_flight batches (1..8, default 4) are on the device at once. */
+    uint32_t batch_max_frames;
+    uint32_t detects_in_flight;
+    uint64_t batch_max_bytes;
 } apsm_server_config;
 typedef struct apsm_frame
 {

@@ -39,7 +39,8 @@ class ApsmParamsC(ctypes.Structure):
 
 
 class ServerConfigC(ctypes.Structure):
-    _fields_ = [("hbm_budget", ctypes.c_uint64), ("frame_arena", ctypes.c_uint64)]
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("frame_arena", ctypes.c_uint64), ("batch_max_frames", ctypes.c_uint32),
+                ("detects_in_flight", ctypes.c_uint32), ("batch_max_bytes", ctypes.c_uint64)]
 
 
 class FrameC(ctypes.Structure):

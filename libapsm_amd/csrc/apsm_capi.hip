@@ -1808,6 +1808,10 @@ int apsm_server_create( const apsm_server_config* cfg, apsm_server** out )
         try
         {
             S->cfg = *cfg;
+            ARG_CHECK( cfg->detects_in_flight <= (uint32_t)kServeDetectInFlightMax, "detects_in_flight must be at most 8" );
+            S->detectSlots = cfg->detects_in_flight ? (int)cfg->detects_in_flight : kServeDetectInFlight;
+            S->batchFrames = cfg->batch_max_frames ? cfg->batch_max_frames : kServeBatchFrames;
+            S->batchBytes = cfg->batch_max_bytes ? (size_t)cfg->batch_max_bytes : kServeBatchBytes;
             S->epoch = std::chrono::steady_clock::now();
             // creation order: detect, upload, train, side (one hardware queue each while the
             // process has no other streams)

@@ -23,7 +23,8 @@
 #include <deque>
 
 constexpr int kServeRoundsInFlight = 2;
-constexpr int kServeDetectInFlight = 4;
+constexpr int kServeDetectInFlightMax = 8; // detect batch slots (apsm_server_config.detects_in_flight <= this)
+constexpr int kServeDetectInFlight = 4; // default
 // One stream each for detect batches, uploads, train rounds and one side stream for a round's
 // further sequencer buckets: four streams, created in that order, so that each gets a hardware
 // queue of its own (a process has four; streams beyond share them round-robin, and a detect
@@ -393,7 +394,9 @@ struct apsm_server
     hipStream_t side[ kServeSideStreams ] = {};
     hipEvent_t fork = nullptr, join[ kServeSideStreams ] = {};
     ServeRound rounds[ kServeRoundsInFlight ];
-    ServeDetect detects[ kServeDetectInFlight ];
+    ServeDetect detects[ kServeDetectInFlightMax ];
+    int detectSlots = kServeDetectInFlight; // batches in flight at most
+    size_t batchFrames = kServeBatchFrames, batchBytes = kServeBatchBytes; // a detect batch closes here
     std::deque<int64_t> roundsInFlight, detectsInFlight; // ids, oldest first
     int64_t nextRound = 0, nextDetect = 0;
     std::deque<ServeFrame*> waiting; // arrival order
@@ -452,7 +455,7 @@ void serverRetire( apsm_server* S )
 {
     while ( !S->detectsInFlight.empty() )
     {
-        ServeDetect& D = S->detects[ S->detectsInFlight.front() % kServeDetectInFlight ];
+        ServeDetect& D = S->detects[ S->detectsInFlight.front() % S->detectSlots ];
         const hipError_t e = hipEventQuery( D.detected );
         if ( e == hipErrorNotReady )
             break;
@@ -486,7 +489,7 @@ void serverRetire( apsm_server* S )
 // it launched one.
 bool serverLaunchDetect( apsm_server* S )
 {
-    if ( S->waiting.empty() || (int)S->detectsInFlight.size() >= kServeDetectInFlight )
+    if ( S->waiting.empty() || (int)S->detectsInFlight.size() >= S->detectSlots )
         return false;
     const double now = serverNow( S );
     // arrival order; a frame whose link is busy waits, a frame without arena room stops admission
@@ -494,7 +497,7 @@ bool serverLaunchDetect( apsm_server* S )
     size_t batchBytes = 0;
     for ( auto it = S->waiting.begin(); it != S->waiting.end(); )
     {
-        if ( admitted.size() >= kServeBatchFrames || batchBytes >= kServeBatchBytes )
+        if ( admitted.size() >= S->batchFrames || batchBytes >= S->batchBytes )
             break;
         ServeFrame* f = *it;
         ServeLink& L = S->links[ f->f.link ];
@@ -526,7 +529,7 @@ bool serverLaunchDetect( apsm_server* S )
         return false;
 
     const int64_t id = S->nextDetect++;
-    ServeDetect& D = S->detects[ id % kServeDetectInFlight ];
+    ServeDetect& D = S->detects[ id % S->detectSlots ];
     D.id = id;
     D.frames = admitted;
     DescArena arena;
@@ -627,7 +630,7 @@ bool serverLaunchTrainRound( apsm_server* S )
     {
         // a batch still in flight: wait for its event (a retired one has completed)
         if ( std::find( S->detectsInFlight.begin(), S->detectsInFlight.end(), b ) != S->detectsInFlight.end() )
-            HIP_OR_THROW( hipStreamWaitEvent( st, S->detects[ b % kServeDetectInFlight ].detected, 0 ) );
+            HIP_OR_THROW( hipStreamWaitEvent( st, S->detects[ b % S->detectSlots ].detected, 0 ) );
     }
     std::vector<apsm::CopyDesc> copies;
     std::vector<apsm::StageDesc> stage;

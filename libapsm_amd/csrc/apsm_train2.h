@@ -195,7 +195,10 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
 
     // window Gram of the rows already in the window (active rows entered before the block);
     // the row in complex slot x, newest first from kLast, is kLast - ((pLast - x) mod Wc)
-    float gA[ TP ], gB[ TP ], gC[ TP ];
+    // vector registers: the entering sample's entry is written at a wave-uniform index by VGPR
+    // index mode (s_set_gpr_idx_on), the product reads them at constant indices
+    typedef float GramRegs __attribute__( ( ext_vector_type( TP ) ) );
+    GramRegs gA, gB, gC;
     {
         const int kLast = a.newRel0 - 1;
         const int pLast = ( ( cMod + kLast ) % Wc + Wc ) % Wc;
@@ -408,20 +411,11 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
                     const float Bv = m * ( later ? g[ 2 ] : g[ 1 ] );
                     const float Cv = m * ( later ? g[ 1 ] : g[ 2 ] );
                     // the active lanes all belong to the part holding slot p, so jj is uniform
-                    // among them: a scalar test per register index (the empty asm keeps each
-                    // write behind its branch instead of a select over every register)
+                    // among them: one indexed register write per array (VGPR index mode)
                     const int jjs = __builtin_amdgcn_readfirstlane( jj );
-#pragma unroll
-                    for ( int j = 0; j < TP; j++ )
-                    {
-                        if ( j == jjs )
-                        {
-                            gA[ j ] = A;
-                            gB[ j ] = Bv;
-                            gC[ j ] = Cv;
-                            asm volatile( "" : "+v"( gA[ j ] ), "+v"( gB[ j ] ), "+v"( gC[ j ] ) );
-                        }
-                    }
+                    gA[ jjs ] = A;
+                    gB[ jjs ] = Bv;
+                    gC[ jjs ] = Cv;
                 }
             }
 #pragma unroll

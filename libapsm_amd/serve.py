@@ -56,8 +56,9 @@ class ApsmServer:
     """hbm_budget: device bytes for link states and the frame arena; frame_arena: device bytes
     for frames in flight (a frame waits until it fits)."""
 
-    def __init__(self, hbm_budget, frame_arena):
-        cfg = ServerConfigC(int(hbm_budget), int(frame_arena))
+    def __init__(self, hbm_budget, frame_arena, batch_max_frames=0, detects_in_flight=0, batch_max_bytes=0):
+        cfg = ServerConfigC(int(hbm_budget), int(frame_arena), int(batch_max_frames), int(detects_in_flight),
+                            int(batch_max_bytes))
         self._s = ctypes.c_void_p()
         check(lib().apsm_server_create(ctypes.byref(cfg), ctypes.byref(self._s)))
         self.links = []  # (antennas, users)
