@@ -253,8 +253,9 @@ int apsm_batch_last_ms(apsm_batch* b, float* ms);
 int apsm_batch_last_split(apsm_batch* b, float* train_ms, float* detect_ms);
 /* Instance i's handle (state readback with apsm_get_state; owned by the batch). */
 apsm_handle* apsm_batch_instance(apsm_batch* b, uint32_t i);
-/* Autotuner choices: rows of (NMAX, UMAX, threads per workgroup, workgroups per CU, work items,
- * kernel: -1 vector ALU, else the matrix-core input precision 0 FP32, 1 BF16, 2 FP16). */
+/* Autotuner choices, one row of 8 ints per detect bucket: (NMAX, UMAX, threads per workgroup,
+ * workgroups per CU, work items, kernel: -1 vector ALU, else the matrix-core input precision
+ * 0 FP32, 1 BF16, 2 FP16, dictionary entries per LDS tile, samples per workgroup tile). */
 int apsm_batch_tuning(apsm_batch* b, int32_t* table, uint32_t cap_rows, uint32_t* rows);
 
 /* Serving engine (no counterpart in the reference, whose tool detects one data set per run).

@@ -82,9 +82,10 @@ class ApsmBatch:
     def tuning(self):
         rows = ctypes.c_uint32()
         check(lib().apsm_batch_tuning(self._b, None, 0, ctypes.byref(rows)))
-        t = np.zeros((max(1, rows.value), 6), np.int32)
+        t = np.zeros((max(1, rows.value), 8), np.int32)
         check(lib().apsm_batch_tuning(self._b, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), t.shape[0],
                                       ctypes.byref(rows)))
         kernels = {-1: "vector", 0: "matrix-fp32", 1: "matrix-bf16", 2: "matrix-fp16"}
         return [dict(nmax=int(r[0]), umax=int(r[1]), threads=int(r[2]), blocks_per_cu=int(r[3]), items=int(r[4]),
-                     kernel=kernels.get(int(r[5]), str(r[5]))) for r in t[:rows.value]]
+                     kernel=kernels.get(int(r[5]), str(r[5])), entries_per_tile=int(r[6]), samples_per_workgroup=int(r[7]))
+                for r in t[:rows.value]]

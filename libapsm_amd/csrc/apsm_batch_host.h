@@ -26,15 +26,17 @@ struct apsm_batch
     size_t trainLaunches = 0;
     hipGraph_t graph[ 2 ] = {};
     hipGraphExec_t exec[ 2 ] = {}; // train, detect
-    std::vector<std::array<int, 6>> tuning; // (NMAX, UMAX, threads, blocks per CU, items, kernel)
+    std::vector<std::array<int, 8>> tuning; // (NMAX, UMAX, threads, blocks per CU, items, kernel, entries per tile, samples per workgroup)
     int cus = 256;
 };
 
 namespace {
 
-// Detect launch shapes the autotuner chooses from: threads per workgroup x workgroups per CU.
+// Detect launch shapes the autotuner chooses from: threads per workgroup (vector-ALU kernel) or
+// tile shape (matrix-core kernel, apsm::kMfmaShape) x workgroups per CU.
 constexpr int kTuneThreads[] = { 128, 256, 512 };
-constexpr int kTuneBlocksPerCu[] = { 2, 4, 8 };
+constexpr int kTuneBlocksPerCu[] = { 1, 2, 4, 8, 16 };
+constexpr int kTuneCands = 5; // >= the thread counts and the matrix-core shapes
 
 template <int NMAX, int UMAX, int THREADS>
 void launchBatchDetect( hipStream_t s, const apsm::DetDesc* d, const apsm::WorkItem* items, int nItems, int grid )
@@ -505,9 +507,9 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
     struct DetBucket
     {
         int kernel, nmax, umax, nCand;
-        int threads[ 3 ];
-        size_t itemOff[ 3 ];
-        int nItems[ 3 ];
+        int threads[ kTuneCands ], shape[ kTuneCands ];
+        size_t itemOff[ kTuneCands ];
+        int nItems[ kTuneCands ];
     };
     std::vector<DetBucket> dbs;
     for ( auto& kv : dBuckets )
@@ -516,11 +518,23 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
         db.kernel = kv.first[ 0 ];
         db.nmax = kv.first[ 1 ];
         db.umax = kv.first[ 2 ];
-        db.nCand = db.kernel < 0 ? 3 : 1;
+        db.nCand = 0;
+        if ( db.kernel < 0 )
+            for ( int t : kTuneThreads )
+            {
+                db.threads[ db.nCand ] = t;
+                db.shape[ db.nCand++ ] = 0;
+            }
+        else
+            for ( int sh = 0; sh < apsm::kMfmaShapes; sh++ )
+                if ( apsm::mfmaShapeBuilt( sh, db.kernel, db.nmax, db.umax ) )
+                {
+                    db.threads[ db.nCand ] = 64 * apsm::kMfmaShape[ sh ].waves;
+                    db.shape[ db.nCand++ ] = sh;
+                }
         for ( int c = 0; c < db.nCand; c++ )
         {
-            db.threads[ c ] = db.kernel < 0 ? kTuneThreads[ c ] : apsm::kMfmaThreads;
-            const int tileSamples = db.kernel < 0 ? kTuneThreads[ c ] : apsm::kMfmaSamples;
+            const int tileSamples = db.kernel < 0 ? db.threads[ c ] : apsm::mfmaShapeSamples( db.shape[ c ] );
             // instances with the largest dictionaries first (a tile's cost is proportional to
             // G): the workgroups of the persistent launch end on cheap tiles, not on a tail of
             // expensive ones
@@ -551,7 +565,8 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
         B->launches[ l ]();
     HIP_OR_THROW( hipStreamSynchronize( B->stream ) );
 
-    // autotune: every (threads, workgroups per CU) candidate per bucket, best of two runs
+    // autotune: every (threads or tile shape, workgroups per CU) candidate per bucket, best of two
+    // runs (launch shapes of the same arithmetic: the results do not depend on the choice)
     std::vector<std::function<void( hipStream_t )>> detLaunches;
     hipEvent_t t0, t1;
     HIP_OR_THROW( hipEventCreate( &t0 ) );
@@ -560,11 +575,11 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
     {
         const apsm::DetDesc* dd = (const apsm::DetDesc*)( base + detOff );
         const int kernel = db.kernel, nmax = db.nmax, umax = db.umax;
-        auto launch = [ = ]( hipStream_t st, int threads, const apsm::WorkItem* it, int nItems, int grid ) {
+        auto launch = [ = ]( hipStream_t st, int threads, int shape, const apsm::WorkItem* it, int nItems, int grid ) {
             if ( kernel < 0 )
                 launchBatchDetectAny( st, nmax, umax, threads, dd, it, nItems, grid );
             else
-                HIP_OR_THROW( apsm::launchBatchDetectMfma( kernel, nmax, umax, st, dd, it, nItems, grid ) );
+                HIP_OR_THROW( apsm::launchBatchDetectMfma( kernel, nmax, umax, st, dd, it, nItems, grid, shape ) );
         };
         float best = 1e30f;
         int bc = 0, bb = 4;
@@ -576,7 +591,8 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
                 for ( int rep = 0; rep < 2; rep++ )
                 {
                     HIP_OR_THROW( hipEventRecord( t0, B->stream ) );
-                    launch( B->stream, db.threads[ c ], (const apsm::WorkItem*)( base + db.itemOff[ c ] ), db.nItems[ c ], grid );
+                    launch( B->stream, db.threads[ c ], db.shape[ c ], (const apsm::WorkItem*)( base + db.itemOff[ c ] ), db.nItems[ c ],
+                            grid );
                     HIP_OR_THROW( hipEventRecord( t1, B->stream ) );
                     HIP_OR_THROW( hipEventSynchronize( t1 ) );
                     float t = 0.f;
@@ -590,11 +606,12 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
                     bb = bpc;
                 }
             }
-        const int threads = db.threads[ bc ], nItems = db.nItems[ bc ];
+        const int threads = db.threads[ bc ], shape = db.shape[ bc ], nItems = db.nItems[ bc ];
         const int grid = std::max( 1, std::min( nItems, B->cus * bb ) );
         const apsm::WorkItem* it = (const apsm::WorkItem*)( base + db.itemOff[ bc ] );
-        detLaunches.push_back( [ = ]( hipStream_t st ) { launch( st, threads, it, nItems, grid ); } );
-        B->tuning.push_back( { nmax, umax, threads, bb, nItems, kernel } );
+        detLaunches.push_back( [ = ]( hipStream_t st ) { launch( st, threads, shape, it, nItems, grid ); } );
+        B->tuning.push_back( { nmax, umax, threads, bb, nItems, kernel, kernel < 0 ? apsm::kDetTile : apsm::mfmaShapeEntries( shape, nmax ),
+                               kernel < 0 ? threads : apsm::mfmaShapeSamples( shape ) } );
     }
     // the buckets run side by side (their tails overlap instead of draining the GPU in turn)
     B->launches.push_back( [ B, detLaunches ]() {

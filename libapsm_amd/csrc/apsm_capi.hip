@@ -1791,8 +1791,8 @@ int apsm_batch_tuning( apsm_batch* B, int32_t* table, uint32_t cap_rows, uint32_
         {
             ARG_CHECK( cap_rows >= B->tuning.size(), "tuning table too small" );
             for ( size_t r = 0; r < B->tuning.size(); r++ )
-                for ( int c = 0; c < 6; c++ )
-                    table[ r * 6 + c ] = B->tuning[ r ][ c ];
+                for ( int c = 0; c < 8; c++ )
+                    table[ r * 8 + c ] = B->tuning[ r ][ c ];
         }
     } );
 }

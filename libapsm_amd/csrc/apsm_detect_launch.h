@@ -18,6 +18,23 @@ constexpr int PREC_F16 = 2;
 constexpr int kMfmaThreads = 256; // 4 waves, 32 samples each
 constexpr int kMfmaSamples = 128; // samples per workgroup tile
 
+// Tile shapes of the persistent batched matrix-core detect, chosen by the batch autotuner:
+// waves per workgroup (32 samples each) and dictionary entries per LDS tile (0: the single
+// handle's choice, 64, or 32 at 64 antennas). Shape 0 is the single-handle shape; the others
+// are built for one-user FP32 instances up to 32 antennas (mfmaShapeBuilt).
+struct MfmaShape
+{
+    int waves, entries;
+};
+constexpr int kMfmaShapes = 5;
+constexpr MfmaShape kMfmaShape[ kMfmaShapes ] = { { 4, 0 }, { 2, 64 }, { 4, 32 }, { 8, 64 }, { 4, 128 } };
+inline bool mfmaShapeBuilt( int shape, int prec, int nmax, int umax )
+{
+    return shape == 0 || ( shape > 0 && shape < kMfmaShapes && prec == PREC_F32 && nmax <= 32 && umax == 1 );
+}
+inline int mfmaShapeSamples( int shape ) { return 32 * kMfmaShape[ shape ].waves; }
+inline int mfmaShapeEntries( int shape, int nmax ) { return kMfmaShape[ shape ].entries ? kMfmaShape[ shape ].entries : ( nmax >= 64 ? 32 : 64 ); }
+
 struct EntryArgs
 {
     int N, L, U, q;
@@ -139,8 +156,9 @@ hipError_t launchBatchEntryCombine( hipStream_t s, const EntryMfmaDesc* d, int n
 hipError_t launchBlockPre( int nmax, int umax, hipStream_t s, const PreDesc& d );
 hipError_t launchBatchBlockPre( int nmax, int umax, hipStream_t s, const PreDesc* d, int n, int gridX );
 
-// Persistent batched detect: `grid` workgroups walk nItems work items of tiles of kMfmaSamples.
+// Persistent batched detect: `grid` workgroups of tile shape `shape` (kMfmaShape) walk nItems
+// work items of tiles of mfmaShapeSamples(shape) samples.
 hipError_t launchBatchDetectMfma( int prec, int nmax, int umax, hipStream_t s, const DetDesc* d, const WorkItem* items,
-                                  int nItems, int grid );
+                                  int nItems, int grid, int shape = 0 );
 
 } // namespace apsm

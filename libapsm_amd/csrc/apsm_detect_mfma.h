@@ -41,14 +41,19 @@ struct MfmaIn<PREC_F16>
     static __device__ __forceinline__ mf32x16 mma( V a, V b, mf32x16 c ) { return __builtin_amdgcn_mfma_f32_32x32x16_f16( a, b, c, 0, 0, 0 ); }
 };
 
-template <int NMAX, int UMAX, int PREC>
+// WAVES: waves per workgroup (32 samples each); ETILE: dictionary entries per LDS tile (0: 64,
+// or 32 at 64 antennas).
+template <int NMAX, int UMAX, int PREC, int WAVES = 4, int ETILE = 0>
 __device__ __forceinline__ void detectTileMfma( const float* __restrict__ rx, int64_t M, const float* __restrict__ Z,
                                                 const float* __restrict__ Zn, const float* __restrict__ C, int64_t ldC,
                                                 const float* __restrict__ theta, int64_t G, int U, float wL, float wG, float g2,
                                                 float* __restrict__ est, int N, const int64_t tile )
 {
     constexpr int K = 2 * NMAX;
-    constexpr int kMfmaTile = NMAX >= 64 ? 32 : 64; // dictionary entries per LDS tile
+    constexpr int kMfmaTile = ETILE ? ETILE : ( NMAX >= 64 ? 32 : 64 ); // dictionary entries per LDS tile
+    constexpr int kMfmaThreads = 64 * WAVES; // (shadows the single-handle shape)
+    constexpr int kMfmaSamples = 32 * WAVES;
+    static_assert( kMfmaTile <= kMfmaThreads && kMfmaTile % 32 == 0, "one entry norm per thread, 32-entry MFMA blocks" );
     // 16-bit inputs: K padded to a multiple of 16 (one MFMA step), rows padded by 8 elements
     // (16 bytes) against LDS bank conflicts of the 16-byte reads
     constexpr int K16 = K < 16 ? 16 : K;
@@ -304,16 +309,18 @@ __global__ void __launch_bounds__( kMfmaThreads ) apsm_detect_mfma( const float*
     detectTileMfma<NMAX, UMAX, PREC>( rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, blockIdx.x );
 }
 
-// Persistent batched detect over a work list (all items of one launch share NMAX and UMAX).
-template <int NMAX, int UMAX, int PREC>
-__global__ void __launch_bounds__( kMfmaThreads ) batch_detect_mfma( const DetDesc* __restrict__ d, const WorkItem* __restrict__ items,
-                                                                    int nItems )
+// Persistent batched detect over a work list (all items of one launch share NMAX and UMAX),
+// workgroups of WAVES waves, tiles of 32 * WAVES samples.
+template <int NMAX, int UMAX, int PREC, int WAVES = 4, int ETILE = 0>
+__global__ void __launch_bounds__( 64 * WAVES ) batch_detect_mfma( const DetDesc* __restrict__ d, const WorkItem* __restrict__ items,
+                                                                   int nItems )
 {
     for ( int w = blockIdx.x; w < nItems; w += gridDim.x )
     {
         const WorkItem it = items[ w ];
         const DetDesc& x = d[ it.desc ];
-        detectTileMfma<NMAX, UMAX, PREC>( x.rx, x.M, x.Z, x.Zn, x.C, x.ldC, x.theta, x.G, x.U, x.wL, x.wG, x.g2, x.est, x.N, it.tile );
+        detectTileMfma<NMAX, UMAX, PREC, WAVES, ETILE>( x.rx, x.M, x.Z, x.Zn, x.C, x.ldC, x.theta, x.G, x.U, x.wL, x.wG, x.g2, x.est,
+                                                         x.N, it.tile );
     }
 }
 
@@ -329,11 +336,30 @@ hipError_t launchMfmaNU( hipStream_t s, const float* rx, int64_t M, const float*
     return hipGetLastError();
 }
 
-template <int PREC, int NMAX, int UMAX>
+template <int PREC, int NMAX, int UMAX, int WAVES = 4, int ETILE = 0>
 hipError_t launchBatchMfmaNU( hipStream_t s, const DetDesc* d, const WorkItem* items, int nItems, int grid )
 {
-    hipLaunchKernelGGL( ( batch_detect_mfma<NMAX, UMAX, PREC> ), dim3( grid ), dim3( kMfmaThreads ), 0, s, d, items, nItems );
+    hipLaunchKernelGGL( ( batch_detect_mfma<NMAX, UMAX, PREC, WAVES, ETILE> ), dim3( grid ), dim3( 64 * WAVES ), 0, s, d, items, nItems );
     return hipGetLastError();
+}
+
+// The batched launch in tile shape `shape` (kMfmaShape); shapes other than 0 exist for one-user
+// FP32 instances up to 32 antennas (mfmaShapeBuilt).
+template <int PREC, int NMAX, int UMAX>
+hipError_t launchBatchMfmaShape( int shape, hipStream_t s, const DetDesc* d, const WorkItem* items, int nItems, int grid )
+{
+    if constexpr ( PREC == PREC_F32 && NMAX <= 32 && UMAX == 1 )
+    {
+        switch ( shape )
+        {
+        case 1: return launchBatchMfmaNU<PREC, NMAX, UMAX, 2, 64>( s, d, items, nItems, grid );
+        case 2: return launchBatchMfmaNU<PREC, NMAX, UMAX, 4, 32>( s, d, items, nItems, grid );
+        case 3: return launchBatchMfmaNU<PREC, NMAX, UMAX, 8, 64>( s, d, items, nItems, grid );
+        case 4: return launchBatchMfmaNU<PREC, NMAX, UMAX, 4, 128>( s, d, items, nItems, grid );
+        default: break;
+        }
+    }
+    return shape == 0 ? launchBatchMfmaNU<PREC, NMAX, UMAX>( s, d, items, nItems, grid ) : hipErrorInvalidValue;
 }
 
 template <int PREC, int NMAX>
@@ -342,23 +368,23 @@ struct MfmaTable
     template <int UMAX>
     static hipError_t one( bool batch, hipStream_t s, const float* rx, int64_t M, const float* Z, const float* Zn, const float* C,
                            int64_t ldC, const float* theta, int64_t G, int U, float wL, float wG, float g2, float* est, int N,
-                           const DetDesc* d, const WorkItem* items, int nItems, int grid )
+                           const DetDesc* d, const WorkItem* items, int nItems, int grid, int shape )
     {
-        return batch ? launchBatchMfmaNU<PREC, NMAX, UMAX>( s, d, items, nItems, grid )
+        return batch ? launchBatchMfmaShape<PREC, NMAX, UMAX>( shape, s, d, items, nItems, grid )
                      : launchMfmaNU<PREC, NMAX, UMAX>( s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N );
     }
     static hipError_t byU( int umax, bool batch, hipStream_t s, const float* rx, int64_t M, const float* Z, const float* Zn,
                            const float* C, int64_t ldC, const float* theta, int64_t G, int U, float wL, float wG, float g2, float* est,
-                           int N, const DetDesc* d, const WorkItem* items, int nItems, int grid )
+                           int N, const DetDesc* d, const WorkItem* items, int nItems, int grid, int shape )
     {
         switch ( umax )
         {
-        case 1: return one<1>( batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
-        case 2: return one<2>( batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
-        case 4: return one<4>( batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
-        case 8: return one<8>( batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
-        case 12: return one<12>( batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
-        case 16: return one<16>( batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
+        case 1: return one<1>( batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid, shape );
+        case 2: return one<2>( batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid, shape );
+        case 4: return one<4>( batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid, shape );
+        case 8: return one<8>( batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid, shape );
+        case 12: return one<12>( batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid, shape );
+        case 16: return one<16>( batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid, shape );
         default: return hipErrorInvalidValue;
         }
     }
@@ -367,15 +393,15 @@ struct MfmaTable
 template <int PREC>
 hipError_t launchMfmaPrec( int nmax, int umax, bool batch, hipStream_t s, const float* rx, int64_t M, const float* Z, const float* Zn,
                            const float* C, int64_t ldC, const float* theta, int64_t G, int U, float wL, float wG, float g2, float* est,
-                           int N, const DetDesc* d, const WorkItem* items, int nItems, int grid )
+                           int N, const DetDesc* d, const WorkItem* items, int nItems, int grid, int shape )
 {
     switch ( nmax )
     {
-    case 4: return MfmaTable<PREC, 4>::byU( umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
-    case 8: return MfmaTable<PREC, 8>::byU( umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
-    case 16: return MfmaTable<PREC, 16>::byU( umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
-    case 32: return MfmaTable<PREC, 32>::byU( umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
-    case 64: return MfmaTable<PREC, 64>::byU( umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
+    case 4: return MfmaTable<PREC, 4>::byU( umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid, shape );
+    case 8: return MfmaTable<PREC, 8>::byU( umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid, shape );
+    case 16: return MfmaTable<PREC, 16>::byU( umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid, shape );
+    case 32: return MfmaTable<PREC, 32>::byU( umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid, shape );
+    case 64: return MfmaTable<PREC, 64>::byU( umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid, shape );
     default: return hipErrorInvalidValue;
     }
 }
@@ -384,7 +410,7 @@ hipError_t launchMfmaPrec( int nmax, int umax, bool batch, hipStream_t s, const 
 #define APSM_MFMA_ENTRY( name )                                                                                                   \
     hipError_t name( int nmax, int umax, bool batch, hipStream_t s, const float* rx, int64_t M, const float* Z, const float* Zn,   \
                      const float* C, int64_t ldC, const float* theta, int64_t G, int U, float wL, float wG, float g2, float* est,  \
-                     int N, const DetDesc* d, const WorkItem* items, int nItems, int grid )
+                     int N, const DetDesc* d, const WorkItem* items, int nItems, int grid, int shape )
 APSM_MFMA_ENTRY( mfmaEntryF32 );
 APSM_MFMA_ENTRY( mfmaEntryBF16 );
 APSM_MFMA_ENTRY( mfmaEntryF16 );

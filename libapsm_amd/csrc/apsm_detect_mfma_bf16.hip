@@ -5,7 +5,8 @@ namespace apsm {
 
 APSM_MFMA_ENTRY( mfmaEntryBF16 )
 {
-    return launchMfmaPrec<PREC_BF16>( nmax, umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
+    return launchMfmaPrec<PREC_BF16>( nmax, umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid,
+                                    shape );
 }
 
 } // namespace apsm

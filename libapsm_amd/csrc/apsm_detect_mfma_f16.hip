@@ -5,7 +5,8 @@ namespace apsm {
 
 APSM_MFMA_ENTRY( mfmaEntryF16 )
 {
-    return launchMfmaPrec<PREC_F16>( nmax, umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
+    return launchMfmaPrec<PREC_F16>( nmax, umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid,
+                                    shape );
 }
 
 } // namespace apsm

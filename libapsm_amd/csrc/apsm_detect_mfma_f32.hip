@@ -6,7 +6,8 @@ namespace apsm {
 
 APSM_MFMA_ENTRY( mfmaEntryF32 )
 {
-    return launchMfmaPrec<PREC_F32>( nmax, umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid );
+    return launchMfmaPrec<PREC_F32>( nmax, umax, batch, s, rx, M, Z, Zn, C, ldC, theta, G, U, wL, wG, g2, est, N, d, items, nItems, grid,
+                                    shape );
 }
 
 hipError_t launchEntryMfma( int nmax, int umax, hipStream_t s, const float* Z, const float* Zn, const float* C, float* part,

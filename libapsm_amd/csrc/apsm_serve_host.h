@@ -553,7 +553,7 @@ bool serverLaunchDetect( apsm_server* S )
     for ( auto& kv : dBuckets )
     {
         const int kernel = kv.first[ 0 ];
-        const int tile = kernel < 0 ? apsm::kDetThreads : apsm::kMfmaSamples;
+        const int tile = kernel < 0 ? apsm::kDetThreads : apsm::mfmaShapeSamples( 0 );
         // frames of the largest dictionaries first (see the batched detect)
         std::vector<int> order( kv.second );
         std::stable_sort( order.begin(), order.end(), [ & ]( int x, int y ) { return det[ x ].G > det[ y ].G; } );
