@@ -471,7 +471,7 @@ void reserveBlockedScratch( apsm_handle* h, int64_t steps )
     const int newPer = (int)( q / 2 );
     h->Pbuf.ensure( (size_t)h->U * steps * newPer * 2 );
     h->Ppart.ensure( (size_t)h->U * ( h->Hc / apsm::kEntryChunk + 2 ) * steps * newPer * 2 );
-    h->Gact.ensure( (size_t)apsm::triIdx( ( h->w2 + steps * q ) / 2 ) * 3 );
+    h->Gact.ensure( (size_t)apsm::gramRow<int64_t>( ( h->w2 + steps * q ) / 2 ) );
     h->EvF.ensure( (size_t)h->U * steps * h->w2 );
 }
 

@@ -77,7 +77,7 @@ struct EntryMfmaDesc
     int nChunks;
 };
 
-// Gram among a block's active samples: Gact[tri(i) + j][3] for active samples j <= i (relative
+// Gram among a block's active samples: Gact[gramRow(i) + 3 j + (0..2)] for active samples j <= i (relative
 // to c0), kappa = wL x.z + wG K in the three real forms A = kappa(a_i,a_j) = kappa(b_i,b_j),
 // B = kappa(a_i,b_j), C = kappa(b_i,a_j); 32 x 32 sample blocks of the lower triangle on the
 // matrix cores, four per workgroup.
@@ -95,6 +95,16 @@ struct GramDesc
     float* Gact;
     GramArgs a;
 };
+
+// Row i of the active Gram (3 (i + 1) floats) starts at gramRow(i) floats: every row is padded
+// to whole 16-byte units, so each starts 16-byte aligned (the sequencer copies rows to LDS in
+// 16-byte lanes). The padding of rows 4m .. 4m+3 is 1, 2, 3, 0 floats; gramRow(nA) floats hold
+// nA rows.
+template <typename I>
+__host__ __device__ inline I gramRow( I i )
+{
+    return 3 * ( i * ( i + 1 ) / 2 ) + 6 * ( i >> 2 ) + ( ( 0x6310 >> ( 4 * ( i & 3 ) ) ) & 15 );
+}
 
 __host__ __device__ inline int64_t gramBlocks( int64_t nA )
 {

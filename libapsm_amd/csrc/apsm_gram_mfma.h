@@ -55,7 +55,7 @@ __device__ __forceinline__ void gramBlockMfma( const float* __restrict__ Z, cons
         return;
     const float g2 = a.g2, k2 = 2.f * g2;
     const float qi = -g2 * Zn[ a.c0 + i ];
-    const int64_t rowBase = i * ( i + 1 ) / 2;
+    const int64_t rowBase = gramRow( i );
 #pragma unroll
     for ( int v = 0; v < 16; v++ )
     {
@@ -67,7 +67,7 @@ __device__ __forceinline__ void gramBlockMfma( const float* __restrict__ Z, cons
         const float E0 = __builtin_amdgcn_exp2f( fminf( fmaf( k2, p, q ), 0.f ) );
         const float E1 = __builtin_amdgcn_exp2f( fminf( fmaf( k2, s, q ), 0.f ) );
         const float E2 = __builtin_amdgcn_exp2f( fminf( fmaf( -k2, s, q ), 0.f ) );
-        float* o = Gact + ( rowBase + j ) * 3;
+        float* o = Gact + rowBase + 3 * j;
         o[ 0 ] = a.wL * p + a.wG * E0;
         o[ 1 ] = a.wL * s + a.wG * E1;
         o[ 2 ] = -a.wL * s + a.wG * E2;

@@ -63,6 +63,23 @@ __device__ __forceinline__ void dmaDword( const float* src, uint32_t ldsWaveBase
     asm volatile( "s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off" ::"s"( ldsWaveBase ), "v"( src ) : "memory" );
 }
 
+// The same with 16 bytes per lane (lane i writes ldsWaveBase + 16*i .. +16; src 16-byte aligned):
+// a quarter of the copy instructions for the same bytes, and the issue cost is per instruction.
+__device__ __forceinline__ void dmaDwordx4( const float* src, uint32_t ldsWaveBase )
+{
+    asm volatile( "s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"( ldsWaveBase ), "v"( src ) : "memory" );
+}
+
+// threadIdx.x behind an empty asm: indices derived from it inside the step loop are recomputed
+// each step (a few integer operations) instead of being hoisted and kept in registers across
+// the loop, for which the register window Gram (3 TP of the 128 registers) leaves no room
+__device__ __forceinline__ int tidInLoop()
+{
+    int t = threadIdx.x;
+    asm volatile( "" : "+v"( t ) );
+    return t;
+}
+
 template <int N>
 __device__ __forceinline__ void waitVm()
 {
@@ -102,7 +119,8 @@ __device__ __forceinline__ void waitVmN( int n )
 // first active real row of user 0.
 // Step package (four buffers in LDS, copied by global->LDS DMA during the product three steps
 // ahead):
-//   [0, NEWPER*3*nAct)       Gram rows of the entering samples (row i at i*3*nAct)
+//   [0, pkRows)              Gram rows of the entering samples (row i at i*rowFloats, rowFloats =
+//                            3*nAct rounded up to 4; pkRows = NEWPER*rowFloats rounded up to 256)
 //   [+0, +2*NEWPER)          their frozen shares P
 //   [+0, +w2)                eviction shares of the window rows
 template <int TP, int NEWPER, int PMAX>
@@ -111,8 +129,11 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
                                           const float* __restrict__ P, const float* __restrict__ Gact, const float* __restrict__ EvF,
                                           const Seq2Args& a, const int u )
 {
-    constexpr int kDpw = ( NEWPER * 3 * kActMax + 2 * NEWPER + kSeqThreads + kSeqThreads - 1 ) / kSeqThreads; // DMAs per wave per step
-    constexpr int kPkCap = kDpw * kSeqThreads; // floats per package buffer
+    // floats per package buffer, in whole kSeqThreads units: the rows (<= NEWPER * 3 * kActMax,
+    // a multiple of 256) and P with the eviction shares (2 * NEWPER + w2, w2 <= 320, rounded up
+    // to 64) fit with room to spare
+    constexpr int kDpw = ( NEWPER * 3 * kActMax + 2 * NEWPER + kSeqThreads + kSeqThreads - 1 ) / kSeqThreads;
+    constexpr int kPkCap = kDpw * kSeqThreads;
     constexpr int kPkBufs = 4; // package buffers: step k's package is issued at step k - 3
     constexpr int TPA = ( TP + 3 ) & ~3; // column segment per part, padded for float4 reads
     constexpr int kColStride = kMaxParts * TPA; // floats per column array (A, B or C)
@@ -123,8 +144,8 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     const int tBase = part * TP;
     const int nAct = a.nActC;
     const int cMod = a.a0modW >> 1; // complex slot of relative complex index 0
-    const int rowFloats = 3 * nAct;
-    const int pkRows = NEWPER * rowFloats;
+    const int rowFloats = ( 3 * nAct + 3 ) & ~3; // package row stride: whole 16-byte units
+    const int pkRows = ( NEWPER * rowFloats + 4 * kWave - 1 ) & ~( 4 * kWave - 1 ); // rows in whole 16-byte-lane copies
     const int pkF = pkRows + 2 * NEWPER + w2;
 
     extern __shared__ float sm[];
@@ -142,37 +163,54 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     float* Cu = C + u * a.ldYC;
     const float* Pu = P + (int64_t)u * a.nNewC * 2;
     const float* EvU = EvF + (int64_t)u * a.nSteps * w2;
-    const float* gEnd = Gact + ( nAct * ( nAct + 1 ) / 2 ) * 3 - 1;
+    const float* gEnd4 = Gact + gramRow( nAct ) - 4; // last 16-byte unit of the block's Gram rows
 
     // issue the package of step kk into buffer kk % kPkBufs. The waves below kColBase (they
     // reduce f and take the product, but do not stage) copy it, after their product, while the
-    // staging waves finish: wave w copies 64-float chunks w, w + kIssueWaves, ... (myChunks)
+    // staging waves finish: wave w copies chunks w, w + kIssueWaves, ... (myChunks). Chunks
+    // [0, nX4) are the Gram rows, 256 floats each (16 bytes per lane); the rest are P and the
+    // eviction shares, 64 floats each.
     constexpr int kIssueWaves = kColBase / kWave;
-    const int nChunks = ( pkF + kWave - 1 ) / kWave;
+    const int nX4 = pkRows / ( 4 * kWave );
+    const int nChunks = nX4 + ( 2 * NEWPER + w2 + kWave - 1 ) / kWave;
     // wave-uniform (readfirstlane): the copy loop and the vmcnt waits below branch on scalars
     const int myChunks = __builtin_amdgcn_readfirstlane( wave < kIssueWaves && wave < nChunks ? ( nChunks - wave + kIssueWaves - 1 ) / kIssueWaves : 0 );
+    // LDS byte address of the package buffers as a 32-bit scalar: the destinations below are
+    // integer offsets from it (a generic pointer here costs a 64-bit register pair, which at
+    // the kernel's 128 registers was spilled and reloaded, with a full vmcnt wait, every step)
+    const uint32_t pkLds = __builtin_amdgcn_readfirstlane( (uint32_t)(uintptr_t)( __attribute__( ( address_space( 3 ) ) ) float* )( sPk ) );
     auto pkIssue = [ & ]( int kk ) {
         const int c0 = a.newRel0 + kk * NEWPER;
-        float* buf = sPk + ( kk % kPkBufs ) * kPkCap;
+        const uint32_t bufLds = pkLds + (uint32_t)( ( kk % kPkBufs ) * kPkCap ) * 4u;
 #pragma unroll 1
         for ( int d = 0; d < myChunks; d++ )
         {
             const int chunk = wave + d * kIssueWaves;
-            const int e = chunk * kWave + lane;
-            const float* src = Gact;
-            if ( e < pkRows )
+            if ( chunk < nX4 )
             {
-                const int i = NEWPER == 1 ? 0 : e / rowFloats;
-                const int t = c0 + i;
-                src = Gact + ( t * ( t + 1 ) / 2 ) * 3 + ( e - i * rowFloats );
-                src = src > gEnd ? gEnd : src;
+                // rows start 16-byte aligned (gramRow) and rowFloats is a multiple of 4, so every
+                // lane's 16 bytes are aligned and within one row (beyond the row's valid 3 (t + 1)
+                // floats they are the next rows' values, which no reader uses)
+                const int e = chunk * 4 * kWave + 4 * lane;
+                const float* src = Gact;
+                if ( e < NEWPER * rowFloats )
+                {
+                    const int i = NEWPER == 1 ? 0 : e / rowFloats;
+                    src = Gact + gramRow( c0 + i ) + ( e - i * rowFloats );
+                    src = src > gEnd4 ? gEnd4 : src;
+                }
+                dmaDwordx4( src, __builtin_amdgcn_readfirstlane( bufLds + (uint32_t)( chunk * 4 * kWave ) * 4u ) );
             }
-            else if ( e < pkRows + 2 * NEWPER )
-                src = Pu + kk * NEWPER * 2 + ( e - pkRows );
-            else if ( e < pkF )
-                src = EvU + kk * w2 + ( e - pkRows - 2 * NEWPER );
-            const uint32_t dst = (uint32_t)(uintptr_t)( __attribute__( ( address_space( 3 ) ) ) float* )( buf + chunk * kWave );
-            dmaDword( src, __builtin_amdgcn_readfirstlane( dst ) );
+            else
+            {
+                const int e = ( chunk - nX4 ) * kWave + lane;
+                const float* src = Pu;
+                if ( e < 2 * NEWPER )
+                    src = Pu + kk * NEWPER * 2 + e;
+                else if ( e < 2 * NEWPER + w2 )
+                    src = EvU + kk * w2 + ( e - 2 * NEWPER );
+                dmaDword( src, __builtin_amdgcn_readfirstlane( bufLds + (uint32_t)( pkRows + ( chunk - nX4 ) * kWave ) * 4u ) );
+            }
         }
     };
 
@@ -216,7 +254,7 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
             const bool ok = gThread && tau < Wc && kLast >= 0 && rt >= 0 && rv >= 0;
             const bool ord = rt >= rv;
             const int hi = ok ? ( ord ? rt : rv ) : 0, lo = ok ? ( ord ? rv : rt ) : 0;
-            const float* g = Gact + ( hi * ( hi + 1 ) / 2 + lo ) * 3;
+            const float* g = Gact + gramRow( hi ) + 3 * lo;
             const float m = ok ? 1.f : 0.f;
             gA[ j ] = m * g[ 0 ];
             gB[ j ] = m * g[ ord ? 1 : 2 ];
@@ -252,7 +290,7 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
 #pragma unroll
             for ( int i = 0; i < NEWPER; i++ )
                 ea[ i ] = eb[ i ] = 0.f;
-            const int jd = tid - kDotBase;
+            const int jd = tidInLoop() - kDotBase;
             if ( jd < c0 )
             {
                 const float ca = sC[ 2 * jd ], cb = sC[ 2 * jd + 1 ];
