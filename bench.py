@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision large-instance measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-small", action="store_true", help="skip the small plain workload (GPU, oracle SER, host cores)")
-    ap.add_argument("--serving-frames", type=int, default=600)
+    ap.add_argument("--serving-frames", type=int, default=1200)
     ap.add_argument("--serving-links", type=int, default=2048)
     ap.add_argument("--only", choices=["batched", "mixed", "serving", "copy"], default=None,
                     help="run one further section alone (for profiling); prints {section: result}; "
@@ -419,6 +419,22 @@ def bench_copy(n_bytes=1 << 30, reps=4):
             "gb_per_s": 2.0 * n_bytes / (ms / 1000.0) / 1e9}
 
 
+def h2d_pinned_gbps(n_bytes=64 << 20, reps=5):
+    """Host-to-device copy rate from pinned host memory (the serving uploads' ceiling), through
+    the library's own pinned and device buffers (synchronous copies of n_bytes). The library's
+    HIP runtime is used, not torch's: torch's wheel carries a HIP runtime of its own, which
+    finds no device once the library's runtime holds it."""
+    from libapsm_amd._lib import DeviceArray, PinnedArray, check, lib
+    a = PinnedArray((n_bytes // 4,), np.float32)
+    a.array[...] = 1.0
+    b = DeviceArray.empty((n_bytes // 4,), np.float32)
+    check(lib().apsm_memcpy_h2d(b.ptr, a.ptr, n_bytes))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        check(lib().apsm_memcpy_h2d(b.ptr, a.ptr, n_bytes))
+    return reps * n_bytes / (time.perf_counter() - t0) / 1e9
+
+
 def bench_serving(n_frames=600, n_links=2048, rates=(1000.0, 2000.0, 4000.0, 8000.0)):
     """Serving loop: up to n_links links (8..32 antennas, QPSK or 16QAM, windowSize 40,
     dictionary 1024) admitted by an HBM budget; frames of 14 OFDM symbols x 12 x U[1, 273]
@@ -475,15 +491,32 @@ def bench_serving(n_frames=600, n_links=2048, rates=(1000.0, 2000.0, 4000.0, 800
     # access to each pinned frame buffer, which costs milliseconds once per buffer; a server
     # reuses a pinned pool, so the timed runs measure the steady state)
     srv.replay(frames(0.0))
-    runs = []
-    for rate in list(rates) + [None]:
+    def run(rate):
         fr = frames(0.0 if rate is None else 1.0 / rate)
         res = srv.replay(fr)
         lat = np.array([r["latency_s"] for r in res]) * 1e3
         span = max(r["output_s"] for r in res) - min(r["arrival_s"] for r in res)
-        runs.append({"offered_frames_per_s": rate, "p50_ms": float(np.percentile(lat, 50)),
-                     "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
-                     "symbols_per_s": symbols / span, "frames_per_s": len(fr) / span})
+        return {"offered_frames_per_s": rate, "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+                "symbols_per_s": symbols / span, "frames_per_s": len(fr) / span}
+
+    runs = [run(rate) for rate in list(rates) + [None]]
+    # the largest offered rate whose p99 latency stays below 1 ms: bisection between the swept
+    # rates that bracket it (five halvings)
+    swept = [r for r in runs if r["offered_frames_per_s"] is not None]
+    under = [r for r in swept if r["p99_ms"] < 1.0]
+    lo = max(under, key=lambda r: r["offered_frames_per_s"]) if under else None
+    over = [r for r in swept if r["p99_ms"] >= 1.0 and (lo is None or r["offered_frames_per_s"] > lo["offered_frames_per_s"])]
+    hi = min(over, key=lambda r: r["offered_frames_per_s"]) if over else None
+    bisect = []
+    if lo is not None and hi is not None:
+        for _ in range(5):
+            mid = run(0.5 * (lo["offered_frames_per_s"] + hi["offered_frames_per_s"]))
+            bisect.append(mid)
+            if mid["p99_ms"] < 1.0:
+                lo = mid
+            else:
+                hi = mid
     errs = sum(datagen.symbol_errors(p["est"].array[0], p["tx_data"], p["bps"]) for p in pay)
     st = srv.stats()
 
@@ -504,6 +537,7 @@ def bench_serving(n_frames=600, n_links=2048, rates=(1000.0, 2000.0, 4000.0, 800
         d.train(p["rx_pilots"].array, p["tx_pilots"].array[0])
     plain_s = time.perf_counter() - t0
     sat = runs[-1]
+    plain_sps = symbols / plain_s
     return {
         "workload": "serving-2048-links",
         "links_admitted": admitted,
@@ -513,6 +547,11 @@ def bench_serving(n_frames=600, n_links=2048, rates=(1000.0, 2000.0, 4000.0, 800
         "frames": n_frames,
         "detected_symbols": int(symbols),
         "rates": runs,
+        "max_rate_p99_under_1ms": lo["offered_frames_per_s"] if lo else None,
+        "max_rate_p99_under_1ms_run": lo,
+        "max_rate_p99_under_1ms_vs_plain": (lo["symbols_per_s"] / plain_sps) if lo else None,
+        "bisection": bisect,
+        "h2d_pinned_GBps": h2d_pinned_gbps(),
         "saturation_symbols_per_s": sat["symbols_per_s"],
         "plain_symbols_per_s": symbols / plain_s,
         "plain_detect_p50_ms": float(np.percentile(np.array(lat_plain) * 1e3, 50)),
