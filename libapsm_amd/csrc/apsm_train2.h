@@ -29,10 +29,9 @@ constexpr int kActMax = 512; // active complex samples per block (host guarantee
 constexpr int kMaxParts = 32; // t-slot parts of the register Gram (1024 / Wc for Wc >= 32)
 constexpr int kRedF = 2048; // per-part f partial slots (>= PMAX * Wc of every instantiation)
 // Thread roles in the sequencer's entering phase: [0, w2) reduce f (w2 <= 320), [kColBase,
-// kColBase + nParts*TP) stage the Gram column (<= 192 slots), [kDotBase, kDotBase + c0) take
-// the active rows' share of the entering f (c0 < kActMax).
+// kColBase + nParts*TP) stage the Gram column (<= 192 slots), the last wave takes the active
+// rows' share of the entering f (c0 < kActMax).
 constexpr int kColBase = 320;
-constexpr int kDotBase = 512;
 
 __host__ __device__ __forceinline__ int64_t triIdx( int64_t i ) { return i * ( i + 1 ) / 2; }
 
@@ -137,7 +136,9 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     constexpr int kPkBufs = 4; // package buffers: step k's package is issued at step k - 3
     constexpr int TPA = ( TP + 3 ) & ~3; // column segment per part, padded for float4 reads
     constexpr int kColStride = kMaxParts * TPA; // floats per column array (A, B or C)
-    const int tid = threadIdx.x, lane = tid & ( kWave - 1 ), wave = tid / kWave;
+    // the wave index as a scalar: everything derived from it stays in scalar registers (the
+    // vector registers are all taken by the window Gram and the step's working set)
+    const int tid = threadIdx.x, lane = tid & ( kWave - 1 ), wave = __builtin_amdgcn_readfirstlane( tid / kWave );
     const int Wc = a.Wc, w2 = a.w2, L = a.L;
     const int v = tid % Wc, part = tid / Wc;
     const bool gThread = part < a.nParts;
@@ -155,7 +156,7 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     float* sC = sB + 2 * kSeqThreads; // [2 * kActMax]
     float* sY = sC + 2 * kActMax; // [2 * kActMax]
     float* sInv = sY + 2 * kActMax; // [kActMax]: 1 / (wL |x|^2 + wG)
-    float* sRedE = sInv + kActMax; // [kSeqWaves][2 * kNewMax]
+    float* sRedE = sInv + kActMax; // [2 * kNewMax] used: the entering samples' active-row shares
     float* sRedF = sRedE + kSeqWaves * 2 * kNewMax; // [2 * kRedF]: per-part f partials (zeros beyond nParts)
     float* sCol = sRedF + 2 * kRedF; // [NEWPER][3][kColStride]
 
@@ -271,35 +272,30 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     auto stage = [ & ]( int ks, int pc0 ) { // pc0: complex slot of the step's first entering sample
         const float* pk = sPk + ( ks % kPkBufs ) * kPkCap;
         const int c0 = a.newRel0 + ks * NEWPER; // first entering complex sample
-        if ( wave >= kDotBase / kWave && wave * kWave - kDotBase >= c0 )
+        if ( wave == kSeqWaves - 1 )
         {
-            // no active row in this wave's range (early steps of a block): its partials are 0
-            if ( lane == 0 )
-            {
-#pragma unroll
-                for ( int i = 0; i < NEWPER; i++ )
-                {
-                    sRedE[ wave * 2 * kNewMax + 2 * i ] = 0.f;
-                    sRedE[ wave * 2 * kNewMax + 2 * i + 1 ] = 0.f;
-                }
-            }
-        }
-        else if ( wave >= kDotBase / kWave )
-        {
+            // the active rows' share of each entering f on the last wave, which holds no part
+            // of the register Gram (the product waves' vector ALUs are the step's bottleneck):
+            // kActMax / kWave active rows per lane, then one cross-lane sum per value
             float ea[ NEWPER ], eb[ NEWPER ];
 #pragma unroll
             for ( int i = 0; i < NEWPER; i++ )
                 ea[ i ] = eb[ i ] = 0.f;
-            const int jd = tidInLoop() - kDotBase;
-            if ( jd < c0 )
-            {
-                const float ca = sC[ 2 * jd ], cb = sC[ 2 * jd + 1 ];
+            const int ln = tidInLoop() & ( kWave - 1 );
 #pragma unroll
-                for ( int i = 0; i < NEWPER; i++ )
+            for ( int r = 0; r < kActMax / kWave; r++ )
+            {
+                const int jd = r * kWave + ln;
+                if ( jd < c0 )
                 {
-                    const float* g = pk + i * rowFloats + 3 * jd;
-                    ea[ i ] = ca * g[ 0 ] + cb * g[ 1 ];
-                    eb[ i ] = ca * g[ 2 ] + cb * g[ 0 ];
+                    const float ca = sC[ 2 * jd ], cb = sC[ 2 * jd + 1 ];
+#pragma unroll
+                    for ( int i = 0; i < NEWPER; i++ )
+                    {
+                        const float* g = pk + i * rowFloats + 3 * jd;
+                        ea[ i ] += ca * g[ 0 ] + cb * g[ 1 ];
+                        eb[ i ] += ca * g[ 2 ] + cb * g[ 0 ];
+                    }
                 }
             }
 #pragma unroll
@@ -308,8 +304,8 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
                 const float va = waveSum( ea[ i ] ), vb = waveSum( eb[ i ] );
                 if ( lane == 0 )
                 {
-                    sRedE[ wave * 2 * kNewMax + 2 * i ] = va;
-                    sRedE[ wave * 2 * kNewMax + 2 * i + 1 ] = vb;
+                    sRedE[ 2 * i ] = va;
+                    sRedE[ 2 * i + 1 ] = vb;
                 }
             }
         }
@@ -384,11 +380,7 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
 #pragma unroll
             for ( int t = 0; t < PMAX; t++ ) // parts beyond nParts hold zeros
                 part[ t ] = sRedF[ 2 * ( t * Wc + cv ) + pt ];
-            constexpr int kDotW0 = kDotBase / kWave;
-            float ent[ kSeqWaves - kDotW0 ];
-#pragma unroll
-            for ( int w = kDotW0; w < kSeqWaves; w++ )
-                ent[ w - kDotW0 ] = sRedE[ w * 2 * kNewMax + e2 ];
+            const float ent = sRedE[ e2 ];
             const float evv = pk[ pkRows + 2 * NEWPER + ( inWin ? wpos : 0 ) ];
             const float pv = pk[ pkRows + e2 ];
             const float fOld = sF[ sl ], yv = sY[ rc ], iv = sInv[ rc >> 1 ], cOld = sC[ rc ];
@@ -405,11 +397,7 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
             {
                 if ( entering )
                 {
-                    float e = pv;
-#pragma unroll
-                    for ( int w = 0; w < kSeqWaves - kDotW0; w++ )
-                        e += ent[ w ];
-                    f = e;
+                    f = pv + ent;
                 }
                 const float res = yv - f;
                 float bb = 0.f;
