@@ -147,7 +147,6 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     const int cMod = a.a0modW >> 1; // complex slot of relative complex index 0
     const int rowFloats = ( 3 * nAct + 3 ) & ~3; // package row stride: whole 16-byte units
     const int pkRows = ( NEWPER * rowFloats + 4 * kWave - 1 ) & ~( 4 * kWave - 1 ); // rows in whole 16-byte-lane copies
-    const int pkF = pkRows + 2 * NEWPER + w2;
 
     extern __shared__ float sm[];
     float* sPk = sm; // [kPkBufs][kPkCap] (first, so the DMA destinations are 256-byte aligned)
@@ -166,12 +165,12 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     const float* EvU = EvF + (int64_t)u * a.nSteps * w2;
     const float* gEnd4 = Gact + gramRow( nAct ) - 4; // last 16-byte unit of the block's Gram rows
 
-    // issue the package of step kk into buffer kk % kPkBufs. The waves below kColBase (they
-    // reduce f and take the product, but do not stage) copy it, after their product, while the
-    // staging waves finish: wave w copies chunks w, w + kIssueWaves, ... (myChunks). Chunks
-    // [0, nX4) are the Gram rows, 256 floats each (16 bytes per lane); the rest are P and the
-    // eviction shares, 64 floats each.
-    constexpr int kIssueWaves = kColBase / kWave;
+    // issue the package of step kk into buffer kk % kPkBufs. Waves 0..3 (one per SIMD; they take
+    // the product but do not stage) copy it after their product, while the staging waves
+    // finish: wave w copies chunks w, w + kIssueWaves, ... (myChunks). Chunks [0, nX4) are the
+    // Gram rows, 256 floats each (16 bytes per lane); the rest are P and the eviction shares,
+    // 64 floats each.
+    constexpr int kIssueWaves = 4;
     const int nX4 = pkRows / ( 4 * kWave );
     const int nChunks = nX4 + ( 2 * NEWPER + w2 + kWave - 1 ) / kWave;
     // wave-uniform (readfirstlane): the copy loop and the vmcnt waits below branch on scalars
