@@ -29,13 +29,13 @@ PEAK_FP32_TFLOPS = 157.3
 
 # HBM bytes of one train call of a workload (all its dispatches: sequencer, block precompute,
 # combine, staging, theta), from two rocprofv3 --pmc passes over `bench.py --steps 1 --warmup 0
-# --no-cpu-baseline --no-lls --no-batched --no-mixed --no-serving`: FETCH_SIZE 6 586 792 KiB
-# x 2.000 (a 1 GiB device copy, `bench.py --only copy`, reads 524 301 KiB of FETCH_SIZE) +
-# WRITE_SIZE 4 601 809 KiB x 1.000 (the copy reads 1 048 576 KiB).
-TRAIN_HBM_BYTES = {"plain-16ant-16qam": (6586792 * 2.000 + 4601809 * 1.000) * 1024}
-# HBM bytes of one apsm_train_seq2 launch (per dispatch, from the same calibrated passes
-# restricted to the sequencer kernel); None until measured.
-SEQ_HBM_BYTES_PER_LAUNCH = {}
+# --no-cpu-baseline --no-lls --no-batched --no-mixed --no-serving --no-small`: FETCH_SIZE
+# 6 603 568 KiB x 2.000 (a 1 GiB device copy, `bench.py --only copy`, reads 524 315 KiB of
+# FETCH_SIZE) + WRITE_SIZE 4 593 643 KiB x 1.000 (the copy reads 1 048 576 KiB).
+TRAIN_HBM_BYTES = {"plain-16ant-16qam": (6603568 * 2.000 + 4593643 * 1.000) * 1024}
+# HBM bytes of one apsm_train_seq2 launch, from the same calibrated passes restricted to the
+# sequencer kernel: FETCH_SIZE 8 717.1 KiB x 2.000 + WRITE_SIZE 53.7 KiB x 1.000 per dispatch.
+SEQ_HBM_BYTES_PER_LAUNCH = {"plain-16ant-16qam": (8717.1 * 2.000 + 53.7 * 1.000) * 1024}
 
 
 def parse():
@@ -661,7 +661,17 @@ def main():
                 "projection_steps_per_launch": T / max(1, seq_n / args.steps),
                 "us_per_projection_step": seq_us / (T / max(1, seq_n / args.steps)),
                 "traffic": SEQ_HBM_BYTES_PER_LAUNCH.get(args.workload),
+                "train_call_traffic": TRAIN_HBM_BYTES.get(args.workload),
                 "share_of_step": seq_ms / args.steps / ms_step}
+    # per-step floor of the chain: the window-Gram product's vector-ALU issue on the busiest SIMD
+    # (4 TP FMA per thread, wave64 at 16 lanes per cycle, packed FP32 off) at 2.4 GHz
+    Wc = w["window"]
+    parts = 1024 // Wc
+    tp = -(-Wc // parts)
+    waves_per_simd = -(-(-(-parts * Wc // 64)) // 4)
+    floor_us = waves_per_simd * 4 * tp * 4 / 2.4e9 * 1e6
+    roofline["per_step_floor_us"] = floor_us
+    roofline["frac_of_per_step_floor"] = floor_us / roofline["us_per_projection_step"]
     det_flops = M * G * (8.0 * N + 8.0 * U) + M * U * 8.0 * N
     det_us = 1000.0 * det_ms_p / max(1, det_n)
     det_tf = det_flops / (det_us * 1e-6) / 1e12
