@@ -107,18 +107,7 @@ void launchBatchSeq2Tp( hipStream_t s, int tp, const apsm::Seq2Desc* d, int n, i
     }
 }
 
-int pmaxOfTp( int tp )
-{
-    switch ( tp )
-    {
-    case 2: return 24;
-    case 4: return 16;
-    case 7: return 12;
-    case 10: return 10;
-    case 16: return 8;
-    default: return 6;
-    }
-}
+int pmaxOfTp( int tp ) { return apsm::seq2PmaxOfTp( tp ); }
 
 // One block of one instance's train as batched-launch descriptors.
 struct TrainBlk

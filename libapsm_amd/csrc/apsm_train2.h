@@ -35,6 +35,26 @@ constexpr int kColBase = 320;
 
 __host__ __device__ __forceinline__ int64_t triIdx( int64_t i ) { return i * ( i + 1 ) / 2; }
 
+// Parts (threads per u-slot) at most for a register-Gram run of TP t-slots: the launchers
+// instantiate the sequencer with this PMAX, and a window of Wc complex slots fits when
+// Wc <= TP * PMAX (the host checks nParts <= PMAX and nParts * TP >= Wc).
+__host__ __device__ constexpr int seq2PmaxOfTp( int tp )
+{
+    return tp == 2 ? 24 : tp == 4 ? 16 : tp == 7 ? 12 : tp == 10 ? 10 : tp == 16 ? 8 : 6;
+}
+
+// Floats per step-package buffer: the Gram rows (<= NEWPER * 3 * kActMax, a multiple of 256)
+// and P with the eviction shares (2 * NEWPER + w2 floats, w2 <= 2 * TP * PMAX, copied in
+// 64-float chunks). Up to TP = 10 (at most 64 registers), rounded up to 256 floats, so that two
+// workgroups fit in a CU's LDS. Beyond, where the registers allow one workgroup per CU anyway,
+// the rows plus a whole kSeqThreads-float spare, in kSeqThreads units (3072 floats at TP = 27:
+// measured 1 % faster there than the tight 2048).
+__host__ __device__ constexpr int seq2PkCap( int newPer, int tp, int pmax )
+{
+    return tp <= 10 ? ( newPer * 3 * kActMax + ( ( 2 * newPer + 2 * tp * pmax + 63 ) & ~63 ) + 255 ) & ~255
+                    : ( newPer * 3 * kActMax + 2 * newPer + 2 * kSeqThreads - 1 ) / kSeqThreads * kSeqThreads;
+}
+
 struct Seq2Args
 {
     int N, L, U, w2, Wc, q, nSteps, hasEv;
@@ -128,11 +148,8 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
                                           const float* __restrict__ P, const float* __restrict__ Gact, const float* __restrict__ EvF,
                                           const Seq2Args& a, const int u )
 {
-    // floats per package buffer, in whole kSeqThreads units: the rows (<= NEWPER * 3 * kActMax,
-    // a multiple of 256) and P with the eviction shares (2 * NEWPER + w2, w2 <= 320, rounded up
-    // to 64) fit with room to spare
-    constexpr int kDpw = ( NEWPER * 3 * kActMax + 2 * NEWPER + kSeqThreads + kSeqThreads - 1 ) / kSeqThreads;
-    constexpr int kPkCap = kDpw * kSeqThreads;
+    static_assert( PMAX == seq2PmaxOfTp( TP ), "sequencer instantiated with another PMAX than its launchers assume" );
+    constexpr int kPkCap = seq2PkCap( NEWPER, TP, PMAX ); // floats per package buffer
     constexpr int kPkBufs = 4; // package buffers: step k's package is issued at step k - 3
     constexpr int TPA = ( TP + 3 ) & ~3; // column segment per part, padded for float4 reads
     constexpr int kColStride = kMaxParts * TPA; // floats per column array (A, B or C)
@@ -557,9 +574,8 @@ __global__ void __launch_bounds__( kSeqThreads ) apsm_train_seq2( const float* _
 // LDS bytes of apsm_train_seq2<TP, NEWPER> for a window of w2 real rows.
 inline size_t seq2LdsBytes( int newPer, int w2, int tp )
 {
-    const int dpw = ( newPer * 3 * kActMax + 2 * newPer + kSeqThreads + kSeqThreads - 1 ) / kSeqThreads;
     const int tpa = ( tp + 3 ) & ~3;
-    const size_t floats = 4 * (size_t)dpw * kSeqThreads + w2 + 2 * kSeqThreads + 4 * kActMax + kActMax + kSeqWaves * 2 * kNewMax
+    const size_t floats = 4 * (size_t)seq2PkCap( newPer, tp, seq2PmaxOfTp( tp ) ) + w2 + 2 * kSeqThreads + 4 * kActMax + kActMax + kSeqWaves * 2 * kNewMax
         + 2 * kRedF + (size_t)newPer * 3 * kMaxParts * tpa;
     return floats * sizeof( float );
 }
