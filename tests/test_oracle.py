@@ -4,8 +4,10 @@ The reference holds no golden vectors, known-answer tests or expected outputs fo
 path (it has no tests at all); what pins the oracle here:
   * two independently organised train algorithms (reference schedule vs incremental
     window-Gram form) that must agree to FP64 rounding;
-  * closed forms: the first projection step from the zero state; the purely linear model
-    (wG = 0) reduces to (p - jq)^T x; the LS solution equals numpy's complex lstsq;
+  * closed forms: the first projection step from the zero state; two steps through the
+    window ramp and from a full-window start; the hyperslab projection (excess over eB
+    halved, rows inside untouched); the purely linear model (wG = 0) reduces to
+    (p - jq)^T x; the LS solution equals numpy's complex lstsq;
   * invariants: dictionary capacity above the stream length changes nothing; two train
     calls equal one call on the concatenation; Cholesky and Householder QR agree;
   * the bundled reference data: LLS and APSM reach the error-free regime on QPSK, which a
@@ -422,3 +424,33 @@ def test_streamed_form_rejects_what_it_does_not_restate(oracle):
         d = small_problem(N=4, U=2, T=50, M=4, seed=1)
         with pytest.raises(ValueError):
             m.train(d["rx_train"], d["tx_train"][:2])
+
+
+@pytest.mark.parametrize("s", [np.complex64(0.8 - 0.6j), np.complex64(-0.5 + 0.004j), np.complex64(0.003 - 0.002j)])
+def test_projection_contracts_the_excess_over_the_hyperslab(oracle, s):
+    # Linear model only (gaussianKernelWeight 0, so wL = 1), one complex pilot, one step from
+    # the zero state with W = 2 real rows: a.b = 0 for the two embedded rows, so each row's
+    # f after the step is beta_t |x|^2 = soft(y_t, eB) / 2. A row outside its hyperslab keeps
+    # its side and halves its excess: |y - f| - eB = (|y| - eB) / 2; a row inside gets beta = 0
+    # (coefficient and share of theta exactly zero).
+    rng = np.random.default_rng(21)
+    N, eB = 4, 0.01
+    x = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
+    cfg = dict(windowSize=1, dictionarySize=4, eB=eB, gaussianKernelVariance=0.3, gaussianKernelWeight=0.0)
+    o = oracle.Apsm(cfg, N)
+    o.train(x[:, None], np.array([s]))
+    st = o.state()
+    f = o.detect(x[:, None])[0]
+    a, b = _embed(x)
+    y = np.array([float(s.real), float(s.imag)])
+    for t in range(2):
+        if abs(y[t]) > eB:
+            r = y[t] - f[t]
+            assert np.sign(r) == np.sign(y[t])
+            assert abs((abs(r) - eB) - (abs(y[t]) - eB) / 2) < 1e-12
+        else:
+            assert st["coeffs"][t] == 0.0
+            assert abs(f[t]) < 1e-15
+    # theta carries exactly the outside rows' coefficients
+    beta = [_soft(y[t], eB) / 2 / (a @ a) for t in range(2)]
+    np.testing.assert_allclose(st["theta"], beta[0] * a + beta[1] * b, rtol=1e-12, atol=1e-15)
