@@ -342,7 +342,6 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
             s2.nActC = nActC;
             s2.newRel0 = (int)( ( sPrev - a0 ) / 2 );
             s2.nFr = (int)( ( a0Next - a0 ) / 2 );
-            s2.pkFloats = newPer * 3 * nActC + 2 * newPer + w2;
             s2.ldYC = ld;
             s2.wL = wL;
             s2.wG = wG;

@@ -598,7 +598,6 @@ void trainBlocked( apsm_handle* h, const StreamBuf& b, int64_t ld, int64_t usedO
         s2.nActC = nActC;
         s2.newRel0 = (int)( ( sPrev - a0 ) / 2 );
         s2.nFr = (int)( ( a0Next - a0 ) / 2 );
-        s2.pkFloats = newPer * 3 * nActC + 2 * newPer + w2;
         s2.ldYC = ld;
         s2.wL = wL;
         s2.wG = wG;

@@ -33,8 +33,6 @@ constexpr int kRedF = 2048; // per-part f partial slots (>= PMAX * Wc of every i
 // rows' share of the entering f (c0 < kActMax).
 constexpr int kColBase = 320;
 
-__host__ __device__ __forceinline__ int64_t triIdx( int64_t i ) { return i * ( i + 1 ) / 2; }
-
 // Parts (threads per u-slot) at most for a register-Gram run of TP t-slots: the launchers
 // instantiate the sequencer with this PMAX, and a window of Wc complex slots fits when
 // Wc <= TP * PMAX (the host checks nParts <= PMAX and nParts * TP >= Wc).
@@ -65,7 +63,6 @@ struct Seq2Args
     int nNewC, nActC; // entering / active complex samples
     int newRel0; // relative complex index of the first entering sample
     int nFr; // complex samples leaving the active set at the block end (thF update)
-    int pkFloats; // floats in one step package
     int64_t ldYC; // user stride of Y and C
     float wL, wG, eB;
 };
