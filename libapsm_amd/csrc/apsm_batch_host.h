@@ -35,7 +35,7 @@ namespace {
 // Detect launch shapes the autotuner chooses from: threads per workgroup (vector-ALU kernel) or
 // tile shape (matrix-core kernel, apsm::kMfmaShape) x workgroups per CU.
 constexpr int kTuneThreads[] = { 128, 256, 512 };
-constexpr int kTuneBlocksPerCu[] = { 1, 2, 4, 8, 16 };
+constexpr int kTuneBlocksPerCu[] = { 1, 2, 4, 8, 16, 32, 64, 128 };
 constexpr int kTuneCands = 5; // >= the thread counts and the matrix-core shapes
 
 template <int NMAX, int UMAX, int THREADS>
