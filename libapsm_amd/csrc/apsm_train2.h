@@ -79,11 +79,13 @@ __device__ __forceinline__ void dmaDword( const float* src, uint32_t ldsWaveBase
     asm volatile( "s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off" ::"s"( ldsWaveBase ), "v"( src ) : "memory" );
 }
 
-// The same with 16 bytes per lane (lane i writes ldsWaveBase + 16*i .. +16; src 16-byte aligned):
-// a quarter of the copy instructions for the same bytes, and the issue cost is per instruction.
-__device__ __forceinline__ void dmaDwordx4( const float* src, uint32_t ldsWaveBase )
+// The same with 16 bytes per lane from a wave-uniform base plus a per-lane 32-bit byte offset
+// (lane i writes ldsWaveBase + 16*i .. +16; base + byteOff 16-byte aligned): a quarter of the
+// copy instructions for the same bytes (the issue cost is per instruction), and the load adds
+// the offset, so no 64-bit address arithmetic runs on the vector ALU
+__device__ __forceinline__ void dmaDwordx4( const float* base, uint32_t byteOff, uint32_t ldsWaveBase )
 {
-    asm volatile( "s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"( ldsWaveBase ), "v"( src ) : "memory" );
+    asm volatile( "s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, %2" ::"s"( ldsWaveBase ), "v"( byteOff ), "s"( base ) : "memory" );
 }
 
 // threadIdx.x behind an empty asm: indices derived from it inside the step loop are recomputed
@@ -177,7 +179,7 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
     float* Cu = C + u * a.ldYC;
     const float* Pu = P + (int64_t)u * a.nNewC * 2;
     const float* EvU = EvF + (int64_t)u * a.nSteps * w2;
-    const float* gEnd4 = Gact + gramRow( nAct ) - 4; // last 16-byte unit of the block's Gram rows
+    const int gEnd4Off = gramRow( nAct ) - 4; // last 16-byte unit of the block's Gram rows
 
     // issue the package of step kk into buffer kk % kPkBufs. Waves 0..3 (one per SIMD; they take
     // the product but do not stage) copy it after their product, while the staging waves
@@ -206,14 +208,14 @@ __device__ __forceinline__ void seq2Body( const float* __restrict__ Z, const flo
                 // lane's 16 bytes are aligned and within one row (beyond the row's valid 3 (t + 1)
                 // floats they are the next rows' values, which no reader uses)
                 const int e = chunk * 4 * kWave + 4 * lane;
-                const float* src = Gact;
+                int off = 0;
                 if ( e < NEWPER * rowFloats )
                 {
                     const int i = NEWPER == 1 ? 0 : e / rowFloats;
-                    src = Gact + gramRow( c0 + i ) + ( e - i * rowFloats );
-                    src = src > gEnd4 ? gEnd4 : src;
+                    off = gramRow( c0 + i ) + ( e - i * rowFloats );
+                    off = off > gEnd4Off ? gEnd4Off : off;
                 }
-                dmaDwordx4( src, __builtin_amdgcn_readfirstlane( bufLds + (uint32_t)( chunk * 4 * kWave ) * 4u ) );
+                dmaDwordx4( Gact, 4u * (uint32_t)off, __builtin_amdgcn_readfirstlane( bufLds + (uint32_t)( chunk * 4 * kWave ) * 4u ) );
             }
             else
             {
