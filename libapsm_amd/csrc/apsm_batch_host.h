@@ -2,8 +2,9 @@
 //
 // A batch owns one detector handle per instance. A run is, for every instance at once: reset,
 // train on its pilots, detect its data samples. Planning walks every instance's train schedule
-// (the same blocks trainBlocked forms), groups the kernels of all instances by round and by
-// template instance, writes one descriptor per (launch, instance) into a single device buffer,
+// (the same blocks trainBlocked forms), groups the kernels of all instances by round (at most
+// one block per instance, packed by the CUs the sequencer workgroups take) and by template
+// instance, writes one descriptor per (launch, instance) into a single device buffer,
 // picks the detect launch shape per bucket by timing candidates on the real batch, and records
 // the launch sequence. A run replays it, captured once into a hipGraph, on the batch's stream.
 #pragma once
@@ -410,19 +411,57 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
                              HIP_OR_THROW( hipGetLastError() );
                          } } );
     }
-    size_t maxBlocks = 0;
-    for ( auto& v : blocks )
-        maxBlocks = std::max( maxBlocks, v.size() );
-    for ( size_t r = 0; r < maxBlocks; r++ )
+    // Rounds: each instance's blocks in order, one per round at most, packed by the CUs their
+    // sequencer workgroups hold (one user's workgroup takes a whole CU, or half of one where two
+    // fit: at most 64 registers and 80 KB of LDS). A round fills up to the device's CUs, the
+    // instances with the most blocks left first: the train ends with its longest chain of blocks
+    // when no round needs a second wave of sequencer workgroups. Instances are independent, so
+    // the order of their blocks across rounds does not change any result.
+    std::vector<std::vector<std::pair<int, int>>> rounds; // (instance, block) per round
+    {
+        std::vector<size_t> next( n, 0 );
+        std::vector<double> weight( n, 0.0 );
+        size_t left = 0;
+        for ( int i = 0; i < n; i++ )
+        {
+            left += blocks[ i ].size();
+            if ( !blocks[ i ].empty() )
+            {
+                const TrainBlk& k = blocks[ i ][ 0 ];
+                const bool two = k.tp <= 10 && k.lds <= ( size_t( 80 ) << 10 );
+                weight[ i ] = ( two ? 0.5 : 1.0 ) * k.s.a.U;
+            }
+        }
+        std::vector<int> order( n );
+        for ( int i = 0; i < n; i++ )
+            order[ i ] = i;
+        while ( left > 0 )
+        {
+            std::stable_sort( order.begin(), order.end(),
+                              [ & ]( int x, int y ) { return blocks[ x ].size() - next[ x ] > blocks[ y ].size() - next[ y ]; } );
+            std::vector<std::pair<int, int>> round;
+            double cap = B->cus;
+            for ( int i : order )
+            {
+                if ( next[ i ] >= blocks[ i ].size() )
+                    break; // sorted: no instance after this one has blocks left
+                if ( weight[ i ] > cap && !round.empty() )
+                    continue;
+                round.push_back( { i, (int)next[ i ]++ } );
+                cap -= weight[ i ];
+                left--;
+            }
+            rounds.push_back( std::move( round ) );
+        }
+    }
+    for ( const auto& round : rounds )
     {
         PreLaunch entry;
         std::map<std::pair<int, int>, std::vector<apsm::Seq2Desc>> seq;
         std::map<std::pair<int, int>, std::pair<int, size_t>> seqDim; // (max users, max lds)
-        for ( int i = 0; i < n; i++ )
+        for ( const auto& ib : round )
         {
-            if ( r >= blocks[ i ].size() )
-                continue;
-            const TrainBlk& k = blocks[ i ][ r ];
+            const TrainBlk& k = blocks[ ib.first ][ ib.second ];
             entry.add( k );
             const auto key = std::make_pair( k.tp, k.newPer );
             seq[ key ].push_back( k.s );
