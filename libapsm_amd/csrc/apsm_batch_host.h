@@ -183,6 +183,35 @@ void launchRoundPre( hipStream_t s, const PreLaunch& pl, const char* base, const
         HIP_OR_THROW( apsm::launchBatchEntryCombine( s, (const apsm::EntryMfmaDesc*)( base + combOff ), nComb, ( pl.maxCombine + 255 ) / 256 ) );
 }
 
+// The same for a batch round: the (NMAX, UMAX) buckets are independent launches, run side by
+// side on the batch's streams (largest NMAX on the main stream), then the combine after them.
+void launchRoundPreForked( apsm_batch* B, const PreLaunch& pl, const char* base,
+                           const std::vector<std::pair<std::pair<int, int>, size_t>>& offs, size_t combOff, int nComb )
+{
+    const int nside = std::min<int>( kBatchSideStreams, (int)offs.size() - 1 );
+    if ( nside > 0 )
+    {
+        HIP_OR_THROW( hipEventRecord( B->fork, B->stream ) );
+        for ( int k = 0; k < nside; k++ )
+            HIP_OR_THROW( hipStreamWaitEvent( B->side[ k ], B->fork, 0 ) );
+    }
+    for ( size_t k = 0; k < offs.size(); k++ )
+    {
+        const auto& o = offs[ offs.size() - 1 - k ];
+        const int lane = (int)( k % ( nside + 1 ) );
+        hipStream_t st = lane == 0 ? B->stream : B->side[ lane - 1 ];
+        HIP_OR_THROW( apsm::launchBatchBlockPre( o.first.first, o.first.second, st, (const apsm::PreDesc*)( base + o.second ),
+                                                 (int)pl.buckets.at( o.first ).size(), pl.maxWg.at( o.first ) ) );
+    }
+    for ( int k = 0; k < nside; k++ )
+    {
+        HIP_OR_THROW( hipEventRecord( B->join[ k ], B->side[ k ] ) );
+        HIP_OR_THROW( hipStreamWaitEvent( B->stream, B->join[ k ], 0 ) );
+    }
+    if ( nComb > 0 )
+        HIP_OR_THROW( apsm::launchBatchEntryCombine( B->stream, (const apsm::EntryMfmaDesc*)( base + combOff ), nComb, ( pl.maxCombine + 255 ) / 256 ) );
+}
+
 void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT, const uint32_t* T, const float* const* rxD, const uint32_t* M,
                 float* const* est, bool useGraph )
 {
@@ -473,7 +502,7 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
             std::vector<std::pair<std::pair<int, int>, size_t>> offs;
             int nComb = 0;
             const size_t combOff = entry.upload( arena, offs, nComb );
-            pend.push_back( { 0, [ = ]( const char* base ) { launchRoundPre( B->stream, entry, base, offs, combOff, nComb ); } } );
+            pend.push_back( { 0, [ = ]( const char* base ) { launchRoundPreForked( B, entry, base, offs, combOff, nComb ); } } );
         }
         // the round's sequencer buckets are independent: fork them onto side streams
         struct SeqLaunch
