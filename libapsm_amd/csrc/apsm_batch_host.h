@@ -443,9 +443,9 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
     // Rounds: each instance's blocks in order, one per round at most, packed by the CUs their
     // sequencer workgroups hold (one user's workgroup takes a whole CU, or half of one where two
     // fit: at most 64 registers and 80 KB of LDS). A round fills up to the device's CUs, the
-    // instances with the most blocks left first: the train ends with its longest chain of blocks
-    // when no round needs a second wave of sequencer workgroups. Instances are independent, so
-    // the order of their blocks across rounds does not change any result.
+    // instances with the most blocks left first, so that the train takes as many rounds as its
+    // longest chain of blocks. Instances are independent, so the order of their blocks across
+    // rounds does not change any result.
     std::vector<std::vector<std::pair<int, int>>> rounds; // (instance, block) per round
     {
         std::vector<size_t> next( n, 0 );
