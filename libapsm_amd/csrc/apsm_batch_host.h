@@ -36,7 +36,9 @@ namespace {
 // Detect launch shapes the autotuner chooses from: threads per workgroup (vector-ALU kernel) or
 // tile shape (matrix-core kernel, apsm::kMfmaShape) x workgroups per CU.
 constexpr int kTuneThreads[] = { 128, 256, 512 };
-constexpr int kTuneBlocksPerCu[] = { 1, 2, 4, 8, 16, 32, 64, 128 };
+// workgroups per CU of the persistent detect; 0: one workgroup per work item (the end of the
+// range: a larger grid than that changes nothing)
+constexpr int kTuneBlocksPerCu[] = { 1, 2, 4, 8, 16, 32, 64, 128, 0 };
 constexpr int kTuneCands = 5; // >= the thread counts and the matrix-core shapes
 
 template <int NMAX, int UMAX, int THREADS>
@@ -640,9 +642,14 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
         float best = 1e30f;
         int bc = 0, bb = 4;
         for ( int c = 0; c < db.nCand; c++ )
+        {
+            int lastGrid = -1;
             for ( int bpc : kTuneBlocksPerCu )
             {
-                const int grid = std::max( 1, std::min( db.nItems[ c ], B->cus * bpc ) );
+                const int grid = std::max( 1, bpc == 0 ? db.nItems[ c ] : std::min( db.nItems[ c ], B->cus * bpc ) );
+                if ( grid == lastGrid )
+                    continue; // the same launch as the previous candidate
+                lastGrid = grid;
                 float ms = 1e30f;
                 for ( int rep = 0; rep < 2; rep++ )
                 {
@@ -659,9 +666,10 @@ void batchPlan( apsm_batch* B, const float* const* rxT, const float* const* txT,
                 {
                     best = ms;
                     bc = c;
-                    bb = bpc;
+                    bb = bpc == 0 ? ( db.nItems[ c ] + B->cus - 1 ) / B->cus : bpc;
                 }
             }
+        }
         const int threads = db.threads[ bc ], shape = db.shape[ bc ], nItems = db.nItems[ bc ];
         const int grid = std::max( 1, std::min( nItems, B->cus * bb ) );
         const apsm::WorkItem* it = (const apsm::WorkItem*)( base + db.itemOff[ bc ] );
